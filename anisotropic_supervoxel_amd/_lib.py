@@ -1,0 +1,100 @@
+"""ctypes binding of libasv.so, the C ABI declared in include/asv.h.
+
+The library is built in-tree (``anisotropic_supervoxel_amd/libasv.so``) by
+``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libasv.so")
+
+ASV_OK = 0
+ERRORS = {-1: "ASV_ERR_ARG", -2: "ASV_ERR_HIP", -3: "ASV_ERR_STATE", -4: "ASV_ERR_NOMEM", -5: "ASV_ERR_ADMIT"}
+
+# Every symbol include/asv.h declares, with (restype, argtypes).
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+SIGNATURES = {
+    "asv_default_params": (None, [_vp, _i32, _i32, _i32, _i32]),
+    "asv_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "asv_destroy": (ctypes.c_int, [_vp]),
+    "asv_seeds_placed": (ctypes.c_int, [_vp]),
+    "asv_grid_step": (ctypes.c_int, [_vp]),
+    "asv_device_bytes": (_i64, [_vp]),
+    "asv_footprint": (_i64, [_vp]),
+    "asv_schedule": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
+    "asv_grid_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
+    "asv_upload_frames": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_upload_flow": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "asv_prepare": (ctypes.c_int, [_vp, _vp]),
+    "asv_seed_stage": (ctypes.c_int, [_vp, _i32, _vp]),
+    "asv_flood": (ctypes.c_int, [_vp, _i32, _vp]),
+    "asv_centroids": (ctypes.c_int, [_vp, _i32, _vp]),
+    "asv_run": (ctypes.c_int, [_vp, _vp]),
+    "asv_run_loop": (ctypes.c_int, [_vp, _vp]),
+    "asv_download_labels": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_download_sums": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_enable_trace": (ctypes.c_int, [_vp, _i32]),
+    "asv_download_trace": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_download_lab": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_download_metric": (ctypes.c_int, [_vp, _vp, _vp]),
+    "asv_run_timed": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "asv_synth_video": (ctypes.c_int, [ctypes.c_uint64, _i32, _i32, _i32, _i32, _f32, _f32, _vp, _vp, _vp, _vp]),
+    "asv_last_error": (ctypes.c_char_p, []),
+    "asv_device_count": (ctypes.c_int, []),
+}
+
+
+class Params(ctypes.Structure):
+    """Mirror of struct asv_params."""
+
+    _fields_ = [
+        ("width", _i32), ("height", _i32), ("depth", _i32),
+        ("seeds_num", _i32), ("lloyd_iters", _i32), ("compactness", _f32),
+        ("tie_rule", _i32), ("angle_gate", _i32), ("schedule", _i32),
+        ("frames_prepared", _i32),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libasv.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class AsvError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != ASV_OK:
+        msg = load().asv_last_error().decode(errors="replace")
+        raise AsvError(f"{ERRORS.get(rc, rc)}: {msg}")
+    return rc
+
+
+def ptr(a):
+    """Data pointer of a C-contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return ctypes.c_void_p(a.ctypes.data)

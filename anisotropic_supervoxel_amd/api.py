@@ -1,0 +1,169 @@
+"""Python host over the C ABI: one `Clip` per video clip.
+
+The stage methods mirror the reference's Image interface
+(GPU-AS/Image.h:71-88) in the order Render::run calls them
+(GPU-AS/Render.cpp:254-319):
+
+    init_gpu           -> Clip(params)                  (asv_create)
+    RGB2LAB + metric   -> Clip.prepare()                (asv_prepare)
+    Ld_first_init /
+    Ld_next_seeds_matrix -> Clip.seed_stage(it)         (asv_seed_stage)
+    flood              -> Clip.flood(it)                (asv_flood)
+    centroids_next_seeds -> Clip.centroids(it)          (asv_centroids)
+    label readback     -> Clip.labels()                 (asv_download_labels)
+
+Errors surface as AsvError carrying the library's message, the way the
+reference prints and returns on failure.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import AsvError, Params, check, ptr  # noqa: F401
+
+
+def default_params(width, height, depth, seeds_num, **kw):
+    """asv_default_params, then keyword overrides (lloyd_iters, compactness,
+    tie_rule, angle_gate, schedule, frames_prepared)."""
+    lib = _lib.load()
+    p = Params()
+    lib.asv_default_params(ctypes.byref(p), width, height, depth, seeds_num)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def seeds_placed_for(width, height, depth, seeds_num):
+    """Grid seeds placed for these sizes (host-side rule, no GPU needed)."""
+    p = default_params(width, height, depth, seeds_num)
+    b = _lib.load().asv_footprint(ctypes.byref(p))
+    if b < 0:
+        raise AsvError("invalid parameters")
+    return p
+
+
+def synth_video(seed, width, height, depth, n_ellipses, vmax=2.0, noise_sigma=4.0):
+    """The benchmark's synthetic video: returns (rgb u8 [D,H,W,3],
+    fwd f32 [D,H,W,2], bwd f32 [D,H,W,2], region i32 [D,H,W]), image rows."""
+    lib = _lib.load()
+    rgb = np.empty((depth, height, width, 3), np.uint8)
+    fwd = np.empty((depth, height, width, 2), np.float32)
+    bwd = np.empty((depth, height, width, 2), np.float32)
+    reg = np.empty((depth, height, width), np.int32)
+    check(lib.asv_synth_video(seed, width, height, depth, n_ellipses, vmax, noise_sigma,
+                              ptr(rgb), ptr(fwd), ptr(bwd), ptr(reg)))
+    return rgb, fwd, bwd, reg
+
+
+class Clip:
+    """One clip's device context (asv_ctx)."""
+
+    def __init__(self, params, stream=None):
+        self.lib = _lib.load()
+        self.p = params
+        self.stream = ctypes.c_void_p(stream) if stream else None
+        h = ctypes.c_void_p()
+        check(self.lib.asv_create(ctypes.byref(params), ctypes.byref(h)))
+        self.h = h
+        self.A = self.lib.asv_seeds_placed(h)
+        self.s = self.lib.asv_grid_step(h)
+        self.shape = (params.depth, params.height, params.width)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.asv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- host-side tables ----------------------------------------------------
+    def schedule(self):
+        steps = np.zeros(64, np.int32)
+        mult = np.zeros(64, np.float32)
+        n = self.lib.asv_schedule(self.h, ptr(steps), ptr(mult), 64)
+        return steps[:n].tolist(), mult[:n].tolist()
+
+    def grid_seeds(self):
+        xyz = np.zeros((self.A, 3), np.int32)
+        self.lib.asv_grid_seeds(self.h, ptr(xyz), xyz.size)
+        return xyz
+
+    def device_bytes(self):
+        return self.lib.asv_device_bytes(self.h)
+
+    # -- inputs --------------------------------------------------------------
+    def upload(self, rgb, fwd, bwd):
+        D, H, W = self.shape
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        fwd = np.ascontiguousarray(fwd, np.float32)
+        bwd = np.ascontiguousarray(bwd, np.float32)
+        if rgb.shape != (D, H, W, 3) or fwd.shape != (D, H, W, 2) or bwd.shape != (D, H, W, 2):
+            raise AsvError("ASV_ERR_ARG: input shapes do not match the clip")
+        check(self.lib.asv_upload_frames(self.h, ptr(rgb), self.stream))
+        check(self.lib.asv_upload_flow(self.h, ptr(fwd), ptr(bwd), self.stream))
+        self._keep = (rgb, fwd, bwd)   # async copies read host memory
+
+    # -- stages --------------------------------------------------------------
+    def prepare(self):
+        check(self.lib.asv_prepare(self.h, self.stream))
+
+    def seed_stage(self, it):
+        check(self.lib.asv_seed_stage(self.h, it, self.stream))
+
+    def flood(self, it):
+        check(self.lib.asv_flood(self.h, it, self.stream))
+
+    def centroids(self, it):
+        check(self.lib.asv_centroids(self.h, it, self.stream))
+
+    def run(self):
+        check(self.lib.asv_run(self.h, self.stream))
+
+    def run_timed(self):
+        ms = np.zeros(5, np.float32)
+        n = ctypes.c_int32(0)
+        check(self.lib.asv_run_timed(self.h, self.stream, ptr(ms), ctypes.byref(n)))
+        return ms, n.value
+
+    def enable_trace(self, on=True):
+        check(self.lib.asv_enable_trace(self.h, 1 if on else 0))
+
+    # -- outputs -------------------------------------------------------------
+    def labels(self):
+        out = np.empty(self.shape, np.int32)
+        check(self.lib.asv_download_labels(self.h, ptr(out), self.stream))
+        return out
+
+    def sums(self):
+        out = np.empty((self.A + 1, 8), np.int64)
+        check(self.lib.asv_download_sums(self.h, ptr(out), self.stream))
+        return out
+
+    def trace(self):
+        out = np.empty((self.p.lloyd_iters, self.A, 12), np.float32)
+        check(self.lib.asv_download_trace(self.h, ptr(out), self.stream))
+        return out
+
+    def lab(self):
+        out = np.empty(self.shape, np.uint32)
+        check(self.lib.asv_download_lab(self.h, ptr(out), self.stream))
+        return out
+
+    def metric(self):
+        out = np.empty(self.shape + (8,), np.float32)
+        check(self.lib.asv_download_metric(self.h, ptr(out), self.stream))
+        return out

@@ -1,0 +1,553 @@
+// asv_capi.cpp -- C ABI of libasv: clip contexts, device buffers and the
+// stage sequence of one plain run (GPU-AS/Render.cpp:254-293), with no host
+// round trip inside the Lloyd loop.
+#include "../../include/asv.h"
+#include "asv_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace asv;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, const char* detail = nullptr)
+{
+    char buf[512];
+    if (detail) std::snprintf(buf, sizeof(buf), fmt, detail);
+    else std::snprintf(buf, sizeof(buf), "%s", fmt);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) return fail(ASV_ERR_HIP, #expr ": %s", hipGetErrorString(e_)); \
+    } while (0)
+
+// Grid seeding, GPU-AS/Image.cpp:20-92 (integer divisions, double pow,
+// float sqrt and truncations as written there).  Returns A or -1.
+int grid_seeds(int W, int H, int D, int K, std::vector<int>* xyz, int* s_out)
+{
+    int V = W * H * D;
+    double voxel_size = (double)(V / K);
+    int s = (int)std::floor(std::pow(voxel_size, 1.0 / 3));
+    int dNum = (int)std::pow((float)(K * D * D / W / H), 1.0 / 3.0);
+    if (dNum == 0) dNum = 1;
+    int frame_seeds = K / dNum;
+    int wNum = (int)std::sqrt((float)(frame_seeds * W / H));
+    if (wNum <= 0) return -1;
+    int hNum = frame_seeds / wNum;
+    if (hNum <= 0) return -1;
+    int stepD = D / dNum, stepH = H / hNum, stepW = W / wNum;
+    int d_rem = D - stepD * dNum, h_rem = H - stepH * hNum, w_rem = W - stepW * wNum;
+    int td = d_rem < 1 ? 0 : 1;
+    int count = 0;
+    for (int k = 0; k < dNum; ++k) {
+        int th = 1;
+        for (int i = 0; i < hNum; ++i) {
+            int tw = 1;
+            for (int j = 0; j < wNum; ++j) {
+                int sx = (int)(j * stepW + 0.5 * stepW + tw);
+                int sy = (int)(i * stepH + 0.5 * stepH + th);
+                int sz = (int)(k * stepD + 0.5 * stepD + td);
+                sx = sx > W - 1 ? W - 1 : sx;
+                sy = sy > H - 1 ? H - 1 : sy;
+                sz = sz > D - 1 ? D - 1 : sz;
+                if (tw < w_rem) ++tw;
+                if (xyz) { xyz->push_back(sx); xyz->push_back(sy); xyz->push_back(sz); }
+                ++count;
+            }
+            if (th < h_rem) ++th;
+        }
+        if (td < d_rem) ++td;
+    }
+    if (s_out) *s_out = s;
+    return count;
+}
+
+// JFA pass list, GPU-AS/Image.cpp:1055-1181 (mode 0) or plain halving from
+// W/2 plus one step-1 pass (mode 1).
+void schedule_list(int mode, int s, int W, std::vector<int>& steps, std::vector<float>& mult)
+{
+    steps.clear();
+    mult.clear();
+    if (mode == ASV_SCHED_REFERENCE) {
+        steps.push_back(1); mult.push_back(1.0f);
+        float k = std::floor(0.5f * (float)s);
+        while (k > 0.0f) {
+            steps.push_back((int)k); mult.push_back(1.0f);
+            if ((int)k == 1) k = 0.0f;
+            k = std::floor(0.5f * k + 0.5f);
+        }
+        k = 2.0f;
+        while (k > 0.0f) {
+            steps.push_back((int)k); mult.push_back(2.0f);
+            if ((int)k == 1) k = 0.0f;
+            k = std::floor(0.5f * k + 0.5f);
+        }
+    } else {
+        float k = std::floor(0.5f * (float)W);
+        while (k > 0.0f) {
+            steps.push_back((int)k); mult.push_back(1.0f);
+            if ((int)k == 1) k = 0.0f;
+            k = std::floor(0.5f * k + 0.5f);
+        }
+        steps.push_back(1); mult.push_back(1.0f);
+    }
+}
+
+// sRGB linearisation table (RGB2LAB.fs:77-90): double pow rounded to float.
+void lin_table(float* lut)
+{
+    for (int c = 0; c < 256; ++c) {
+        float u = (float)c / 255.0f;
+        if (u <= 0.04045f) lut[c] = u / 12.92f;
+        else {
+            float t = (u + 0.055f) / 1.055f;
+            lut[c] = (float)std::pow((double)t, 2.4);
+        }
+    }
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count, int64_t* total)
+{
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = 16;
+    *total += (int64_t)bytes;
+    return hipMalloc((void**)p, bytes);
+}
+
+bool valid_params(const asv_params* p)
+{
+    if (!p) return false;
+    if (p->width < 2 || p->height < 2 || p->depth < 1) return false;
+    if ((int64_t)p->width * p->height * p->depth > (int64_t)1 << 30) return false;
+    if (p->seeds_num < 1 || p->lloyd_iters < 1) return false;
+    if (!(p->compactness > 0.0f)) return false;
+    if (p->tie_rule != ASV_TIE_LOWEST_ID && p->tie_rule != ASV_TIE_SCAN_ORDER) return false;
+    if (p->schedule != ASV_SCHED_REFERENCE && p->schedule != ASV_SCHED_HALVING_PLUS_ONE) return false;
+    return true;
+}
+
+}  // namespace
+
+struct asv_ctx {
+    asv_params p{};
+    Dims d{};
+    int s = 0;
+    float ratio = 0.0f;
+    std::vector<int> steps;
+    std::vector<float> mult;
+    std::vector<int> grid_host;
+    int64_t bytes = 0;
+    bool have_frames = false, have_flow = false, prepared = false;
+    bool trace_on = false;
+    int cur = 0;            // index of the label buffer holding the latest labels
+    int last_iter = 0;
+
+    uint8_t* rgb = nullptr;
+    float2* fwd = nullptr;
+    float2* bwd = nullptr;
+    float* lut = nullptr;
+    uint32_t* lab = nullptr;
+    float4* field = nullptr;
+    int* fix_list = nullptr;
+    int* fix_count = nullptr;
+    int32_t* labels[2] = {nullptr, nullptr};
+    int4* grid = nullptr;
+    int4* pos = nullptr;
+    float4* slab = nullptr;
+    SeedRec* rec = nullptr;
+    unsigned long long* sums[2] = {nullptr, nullptr};
+    float* trace = nullptr;
+
+    int64_t V() const { return (int64_t)d.W * d.H * d.D; }
+};
+
+namespace {
+
+void free_ctx(asv_ctx* c)
+{
+    void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
+                    c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
+                    c->trace};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+}
+
+int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace)
+{
+    int64_t b = 0;
+    b += V * 3;                  // rgb
+    b += V * 16;                 // fwd + bwd
+    b += V * 4;                  // lab
+    b += V * 32;                 // metric field
+    b += V * 4;                  // fixup list
+    b += V * 8;                  // labels x2
+    b += A * (16 * 3 + 48);      // grid, pos, slab, rec
+    b += (A + 1) * 64 * 2;       // sums x2
+    if (trace) b += (int64_t)iters * A * 48;
+    return b + 4096;
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int check_ready(asv_ctx* c)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (!c->have_frames) return fail(ASV_ERR_STATE, "frames not uploaded");
+    if (!c->have_flow) return fail(ASV_ERR_STATE, "flow not uploaded");
+    return ASV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void asv_default_params(asv_params* p, int32_t width, int32_t height, int32_t depth, int32_t seeds_num)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->width = width;
+    p->height = height;
+    p->depth = depth;
+    p->seeds_num = seeds_num;
+    p->lloyd_iters = 20;             // GPU-AS/Render.cpp:11
+    p->compactness = 8.0f;           // GPU-AS/Image.cpp:1050
+    p->tie_rule = ASV_TIE_LOWEST_ID;
+    p->angle_gate = 1;
+    p->schedule = ASV_SCHED_REFERENCE;
+    p->frames_prepared = 0;
+}
+
+const char* asv_last_error(void) { return g_err.c_str(); }
+
+int asv_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int64_t asv_footprint(const asv_params* p)
+{
+    if (!valid_params(p)) return -1;
+    int s = 0;
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, nullptr, &s);
+    if (A <= 0) return -1;
+    return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false);
+}
+
+int asv_create(const asv_params* p, asv_ctx** out)
+{
+    if (!out) return fail(ASV_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    asv_ctx* c = new asv_ctx();
+    c->p = *p;
+    int s = 0;
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &c->grid_host, &s);
+    if (A <= 0) { delete c; return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes"); }
+    if (s < 1) s = 1;
+    c->s = s;
+    c->d = Dims{p->width, p->height, p->depth, A};
+    c->ratio = (p->compactness / (float)s) * (p->compactness / (float)s);   // Image.cpp:1051
+    schedule_list(p->schedule, s, p->width, c->steps, c->mult);
+
+    const int64_t V = c->V();
+    hipError_t e = hipSuccess;
+    int64_t& tb = c->bytes;
+    if (e == hipSuccess) e = dalloc(&c->rgb, (size_t)V * 3, &tb);
+    if (e == hipSuccess) e = dalloc(&c->fwd, (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->bwd, (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->lut, 256, &tb);
+    if (e == hipSuccess) e = dalloc(&c->lab, (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->field, (size_t)V * 2, &tb);
+    if (e == hipSuccess) e = dalloc(&c->fix_list, (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->fix_count, 4, &tb);
+    if (e == hipSuccess) e = dalloc(&c->labels[0], (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->labels[1], (size_t)V, &tb);
+    if (e == hipSuccess) e = dalloc(&c->grid, (size_t)A, &tb);
+    if (e == hipSuccess) e = dalloc(&c->pos, (size_t)A, &tb);
+    if (e == hipSuccess) e = dalloc(&c->slab, (size_t)A, &tb);
+    if (e == hipSuccess) e = dalloc(&c->rec, (size_t)A, &tb);
+    if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(A + 1) * kSumWords, &tb);
+    if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(A + 1) * kSumWords, &tb);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        delete c;
+        return fail(ASV_ERR_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    float lut[256];
+    lin_table(lut);
+    std::vector<int4> g((size_t)A);
+    for (int i = 0; i < A; ++i)
+        g[i] = make_int4(c->grid_host[3 * i], c->grid_host[3 * i + 1], c->grid_host[3 * i + 2], 0);
+    e = hipMemcpy(c->lut, lut, sizeof(lut), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c->grid, g.data(), sizeof(int4) * (size_t)A, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        delete c;
+        return fail(ASV_ERR_HIP, "upload of tables failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return ASV_OK;
+}
+
+int asv_destroy(asv_ctx* c)
+{
+    if (!c) return ASV_OK;
+    free_ctx(c);
+    delete c;
+    return ASV_OK;
+}
+
+int asv_seeds_placed(const asv_ctx* c) { return c ? c->d.A : fail(ASV_ERR_ARG, "null context"); }
+int asv_grid_step(const asv_ctx* c) { return c ? c->s : fail(ASV_ERR_ARG, "null context"); }
+int64_t asv_device_bytes(const asv_ctx* c) { return c ? c->bytes : -1; }
+
+int asv_schedule(const asv_ctx* c, int32_t* steps, float* mult, int32_t cap)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    int n = (int)c->steps.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        if (steps) steps[i] = c->steps[i];
+        if (mult) mult[i] = c->mult[i];
+    }
+    return n;
+}
+
+int asv_grid_seeds(const asv_ctx* c, int32_t* xyz, int32_t cap)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    int n = c->d.A;
+    for (int i = 0; i < n * 3 && i < cap; ++i) xyz[i] = c->grid_host[i];
+    return n;
+}
+
+int asv_upload_frames(asv_ctx* c, const uint8_t* rgb, void* stream)
+{
+    if (!c || !rgb) return fail(ASV_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->rgb, rgb, (size_t)c->V() * 3, hipMemcpyDefault, S(stream)));
+    c->have_frames = true;
+    c->prepared = false;
+    return ASV_OK;
+}
+
+int asv_upload_flow(asv_ctx* c, const float* fwd, const float* bwd, void* stream)
+{
+    if (!c || !fwd || !bwd) return fail(ASV_ERR_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(c->fwd, fwd, (size_t)c->V() * 8, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipMemcpyAsync(c->bwd, bwd, (size_t)c->V() * 8, hipMemcpyDefault, S(stream)));
+    c->have_flow = true;
+    c->prepared = false;
+    return ASV_OK;
+}
+
+int asv_prepare(asv_ctx* c, void* stream)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    hipStream_t st = S(stream);
+    HIP_TRY(launch_zero_i32(st, c->fix_count, 1));
+    HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->fwd, c->bwd, c->lut, c->lab, c->field,
+                           c->fix_list, c->fix_count));
+    HIP_TRY(launch_fixup(st, c->d, c->fwd, c->bwd, c->field, c->fix_list, c->fix_count));
+    c->prepared = true;
+    return ASV_OK;
+}
+
+int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run on these inputs");
+    if (iter < 1 || iter > c->p.lloyd_iters) return fail(ASV_ERR_ARG, "iteration out of range");
+    if (iter != 1 && c->last_iter != iter - 1) return fail(ASV_ERR_STATE, "seed stage out of order");
+    hipStream_t st = S(stream);
+    if (iter == 1) {
+        HIP_TRY(launch_seed_init(st, c->d, c->grid, c->lab, c->pos, c->slab));
+        HIP_TRY(launch_zero_u64(st, reinterpret_cast<uint64_t*>(c->sums[1]), (int64_t)(c->d.A + 1) * kSumWords));
+        c->cur = 0;
+    }
+    float* tr = c->trace_on && c->trace ? c->trace + (size_t)(iter - 1) * c->d.A * 12 : nullptr;
+    int gate = (iter >= 2 && c->p.angle_gate) ? 1 : 0;
+    HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->field, c->fwd, gate, c->rec, tr));
+    HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
+    HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
+    c->last_iter = iter;
+    return ASV_OK;
+}
+
+int asv_flood(asv_ctx* c, int32_t iter, void* stream)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (iter != c->last_iter) return fail(ASV_ERR_STATE, "flood before the seed stage of this iteration");
+    hipStream_t st = S(stream);
+    for (size_t i = 0; i < c->steps.size(); ++i) {
+        HIP_TRY(launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i],
+                           c->mult[i] * c->ratio, c->p.tie_rule));
+        c->cur = 1 - c->cur;
+    }
+    return ASV_OK;
+}
+
+int asv_centroids(asv_ctx* c, int32_t iter, void* stream)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (iter != c->last_iter) return fail(ASV_ERR_STATE, "centroids before the seed stage of this iteration");
+    hipStream_t st = S(stream);
+    unsigned long long* s = c->sums[iter & 1];
+    HIP_TRY(launch_reduce(st, c->d, c->labels[c->cur], c->lab, s));
+    if (iter != c->p.lloyd_iters)                                  // GPU-AS/Image.cpp:1269
+        HIP_TRY(launch_finalize(st, c->d, s, c->sums[(iter + 1) & 1], c->lab, c->pos, c->slab));
+    return ASV_OK;
+}
+
+int asv_run_loop(asv_ctx* c, void* stream)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run on these inputs");
+    for (int it = 1; it <= c->p.lloyd_iters; ++it) {
+        if ((r = asv_seed_stage(c, it, stream))) return r;
+        if ((r = asv_flood(c, it, stream))) return r;
+        if ((r = asv_centroids(c, it, stream))) return r;
+    }
+    return ASV_OK;
+}
+
+int asv_run(asv_ctx* c, void* stream)
+{
+    int r = asv_prepare(c, stream);
+    if (r) return r;
+    return asv_run_loop(c, stream);
+}
+
+int asv_run_timed(asv_ctx* c, void* stream, float* ms, int32_t* passes)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!ms) return fail(ASV_ERR_ARG, "null timing output");
+    hipStream_t st = S(stream);
+    // Two events per launch group; recorded on the launch stream itself.
+    std::vector<hipEvent_t> ev;
+    auto mark = [&](void) -> hipEvent_t {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev.push_back(e);
+        (void)hipEventRecord(e, st);
+        return e;
+    };
+    struct Span { hipEvent_t a, b; int cls; };
+    std::vector<Span> spans;
+    hipEvent_t t0 = mark();
+    hipEvent_t a = mark();
+    if ((r = asv_prepare(c, stream))) return r;
+    spans.push_back({a, mark(), 1});
+    int np = 0;
+    for (int it = 1; it <= c->p.lloyd_iters; ++it) {
+        a = mark();
+        if ((r = asv_seed_stage(c, it, stream))) return r;
+        spans.push_back({a, mark(), 4});
+        for (size_t i = 0; i < c->steps.size(); ++i) {
+            a = mark();
+            HIP_TRY(launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i],
+                               c->mult[i] * c->ratio, c->p.tie_rule));
+            spans.push_back({a, mark(), 2});
+            c->cur = 1 - c->cur;
+            ++np;
+        }
+        unsigned long long* s = c->sums[it & 1];
+        a = mark();
+        HIP_TRY(launch_reduce(st, c->d, c->labels[c->cur], c->lab, s));
+        spans.push_back({a, mark(), 3});
+        if (it != c->p.lloyd_iters) {
+            a = mark();
+            HIP_TRY(launch_finalize(st, c->d, s, c->sums[(it + 1) & 1], c->lab, c->pos, c->slab));
+            spans.push_back({a, mark(), 4});
+        }
+    }
+    hipEvent_t t1 = mark();
+    HIP_TRY(hipEventSynchronize(t1));
+    for (int i = 0; i < 5; ++i) ms[i] = 0.0f;
+    float f = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&f, t0, t1));
+    ms[0] = f;
+    for (auto& sp : spans) {
+        HIP_TRY(hipEventElapsedTime(&f, sp.a, sp.b));
+        ms[sp.cls] += f;
+    }
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (passes) *passes = np;
+    return ASV_OK;
+}
+
+int asv_download_labels(asv_ctx* c, int32_t* labels, void* stream)
+{
+    if (!c || !labels) return fail(ASV_ERR_ARG, "null argument");
+    if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
+    HIP_TRY(hipMemcpyAsync(labels, c->labels[c->cur], (size_t)c->V() * 4, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+int asv_download_sums(asv_ctx* c, int64_t* sums, void* stream)
+{
+    if (!c || !sums) return fail(ASV_ERR_ARG, "null argument");
+    if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
+    HIP_TRY(hipMemcpyAsync(sums, c->sums[c->last_iter & 1], (size_t)(c->d.A + 1) * kSumWords * 8,
+                           hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+int asv_enable_trace(asv_ctx* c, int32_t on)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (on && !c->trace) {
+        size_t n = (size_t)c->p.lloyd_iters * c->d.A * 12;
+        hipError_t e = hipMalloc((void**)&c->trace, n * sizeof(float));
+        if (e != hipSuccess) { c->trace = nullptr; return fail(ASV_ERR_NOMEM, "trace allocation failed"); }
+        c->bytes += (int64_t)(n * sizeof(float));
+    }
+    c->trace_on = on != 0;
+    return ASV_OK;
+}
+
+int asv_download_trace(asv_ctx* c, float* trace, void* stream)
+{
+    if (!c || !trace) return fail(ASV_ERR_ARG, "null argument");
+    if (!c->trace) return fail(ASV_ERR_STATE, "trace not enabled");
+    size_t n = (size_t)c->p.lloyd_iters * c->d.A * 12;
+    HIP_TRY(hipMemcpyAsync(trace, c->trace, n * sizeof(float), hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+int asv_download_lab(asv_ctx* c, uint32_t* lab, void* stream)
+{
+    if (!c || !lab) return fail(ASV_ERR_ARG, "null argument");
+    if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run");
+    HIP_TRY(hipMemcpyAsync(lab, c->lab, (size_t)c->V() * 4, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+int asv_download_metric(asv_ctx* c, float* m8, void* stream)
+{
+    if (!c || !m8) return fail(ASV_ERR_ARG, "null argument");
+    if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run");
+    HIP_TRY(hipMemcpyAsync(m8, c->field, (size_t)c->V() * 32, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// asv_internal.h -- device data layout and kernel launchers shared by the
+// C ABI (asv_capi.cpp) and the kernels (asv_kernels.hip).  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace asv {
+
+// One seed as the flood reads it: 48 bytes, three 16-byte loads.
+//   a = (x, y, z, m0)   voxel coordinates as exact floats
+//   b = (m1, m2, m3, m4)
+//   c = (m5, L, a, b)
+struct alignas(16) SeedRec {
+    float4 a, b, c;
+};
+
+// Per-slot centroid sums: sum(2x+1), sum(2y+1), sum(2z+1), n, sum L, a, b, pad.
+constexpr int kSumWords = 8;
+
+// Geometry of one clip.
+struct Dims {
+    int W, H, D;
+    int A;                  // seeds placed
+};
+
+// Launchers (all asynchronous on `st`).  Return hipSuccess or the launch error.
+hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared,
+                          const float2* fwd, const float2* bwd, const float* lin_lut,
+                          uint32_t* lab, float4* field, int* fix_list, int* fix_count);
+hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd,
+                        float4* field, const int* fix_list, const int* fix_count);
+hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, const uint32_t* lab,
+                            int4* pos, float4* slab);
+hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab,
+                              const float4* field, const float2* fwd, int gate, SeedRec* rec,
+                              float* trace_iter);
+hipError_t launch_fill(hipStream_t st, int32_t* p, int64_t n, int32_t value);
+hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n);
+hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
+hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
+hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule);
+hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
+                         unsigned long long* sums);
+hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long long* sums,
+                           unsigned long long* zero_next, const uint32_t* lab, int4* pos,
+                           float4* slab);
+
+}  // namespace asv

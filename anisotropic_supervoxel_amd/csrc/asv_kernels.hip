@@ -1,0 +1,765 @@
+// asv_kernels.hip -- gfx950 kernels of the supervoxel hot path.
+//
+// Arithmetic follows one fixed specification shared with the CPU oracle:
+// no contraction (built with -ffp-contract=off), IEEE division and square
+// root, exact integer centroid sums, lowest-seed-id tie breaking (or the
+// reference's scan order).  Every kernel is a streaming or gather kernel over
+// int32 labels and packed u8 LAB: none of it is a dense contraction, so there
+// is no MFMA work here; the bound is HBM bandwidth.
+//
+// Reference passes replaced (paths relative to the reference root):
+//   k_prepare / k_fixup   shaders/RGB2LAB.fs:11-79 + Image.cpp:874-1009
+//   k_seed_init           shaders/draw_seeds.vs:20-35 (colour at the seed voxel)
+//   k_seed_metric         Image.cpp:689-872 and :1351-1534
+//   k_fill + k_place      draw_seeds / draw_next_seeds (clear -1, later id wins)
+//   k_jfa                 shaders/flood.gs:53-209
+//   k_reduce              shaders/centroids.vs:19-53 with additive blending
+//   k_finalize            shaders/aver_exception.vs:22-81
+
+#include "asv_internal.h"
+
+namespace asv {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+__device__ __forceinline__ int reflect101(int i, int n)
+{
+    if (n == 1) return 0;
+    if (i < 0) return -i;
+    if (i >= n) return 2 * n - 2 - i;
+    return i;
+}
+
+// ---------------------------------------------------------------------------
+// LAB (RGB2LAB.fs:11-79)
+// ---------------------------------------------------------------------------
+
+// Cube root on (0, 1.1] by five Halley steps in double from a linear start.
+__device__ __forceinline__ double cbrt_halley(double x)
+{
+    double y = 0.45 + 0.55 * x;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double y3 = y * y * y;
+        y = y * ((y3 + 2.0 * x) / (2.0 * y3 + x));
+    }
+    return y;
+}
+
+__device__ __forceinline__ float lab_f(float r)
+{
+    if (r > 0.008856f) return (float)cbrt_halley((double)r);
+    return (903.3f * r + 16.0f) / 116.0f;
+}
+
+__device__ __forceinline__ uint32_t round_u8(float v)
+{
+    float r = floorf(v + 0.5f);
+    r = (r >= 0.0f) ? r : 0.0f;
+    r = (r > 255.0f) ? 255.0f : r;
+    return (uint32_t)r;
+}
+
+__device__ __forceinline__ uint32_t lab_pack(uint32_t r, uint32_t g, uint32_t b, const float* lin)
+{
+    float tr = lin[r], tg = lin[g], tb = lin[b];
+    float X = (tr * 0.412453f + tg * 0.357580f) + tb * 0.180423f;
+    float Y = (tr * 0.212671f + tg * 0.715160f) + tb * 0.072169f;
+    float Z = (tr * 0.019334f + tg * 0.119193f) + tb * 0.950227f;
+    float fx = lab_f(X / 0.950456f);
+    float fy = lab_f(Y / 1.0f);
+    float fz = lab_f(Z / 1.088754f);
+    float L = 116.0f * fy - 16.0f;
+    float A = 500.0f * (fx - fy);
+    float B = 200.0f * (fy - fz);
+    return round_u8(L * 2.55f) | (round_u8(A + 128.0f) << 8) | (round_u8(B + 128.0f) << 16);
+}
+
+// ---------------------------------------------------------------------------
+// per-voxel metric (Image.cpp:874-1009)
+// ---------------------------------------------------------------------------
+
+__device__ __constant__ int c_expected_b[9] = {3, 2, 3, 2, 1, 2, 3, 2, 3};
+
+// Upper triangle of N = A^T A (21 entries, row-major i<=j) and c = A^T b,
+// accumulated row by row in the reference's neighbour order (offset_y outer,
+// offset_x inner, forward row before backward row).  Entries are exact
+// integers.  Returns the row count.
+__device__ __forceinline__ int normal_equations(const float2* __restrict__ fwd, const float2* __restrict__ bwd,
+                                                int W, int H, int D, int x, int y, int t,
+                                                double N[21], double c[6])
+{
+#pragma unroll
+    for (int i = 0; i < 21; ++i) N[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c[i] = 0.0;
+    int rows = 0;
+    const size_t frame = (size_t)t * H * W;
+    for (int m = -1; m <= 1; ++m) {
+        for (int n = -1; n <= 1; ++n) {
+            int nx = x + n, ny = y + m;
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+            double b = (double)c_expected_b[(m + 1) * 3 + (n + 1)];
+            size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;   // image row of GL row ny
+            for (int dir = 0; dir < 2; ++dir) {
+                if (dir == 0 && t == D - 1) continue;
+                if (dir == 1 && t == 0) continue;
+                float2 f = dir == 0 ? fwd[pix] : bwd[pix];
+                float fu = f.x, fv = -f.y;
+                float dx = (float)n + floorf(fu + 0.5f);
+                float dy = (float)m + floorf(fv + 0.5f);
+                double sgn = dir == 0 ? 2.0 : -2.0;
+                double r[6];
+                r[0] = (double)dx * (double)dx;
+                r[1] = 2.0 * (double)dx * (double)dy;
+                r[2] = sgn * (double)dx;
+                r[3] = (double)dy * (double)dy;
+                r[4] = sgn * (double)dy;
+                r[5] = 1.0;
+                int q = 0;
+#pragma unroll
+                for (int i = 0; i < 6; ++i) {
+#pragma unroll
+                    for (int j = i; j < 6; ++j) { N[q] += r[i] * r[j]; ++q; }
+                    c[i] += r[i] * b;
+                }
+                ++rows;
+            }
+        }
+    }
+    return rows;
+}
+
+__device__ __forceinline__ int tri(int i, int j) { return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j); }
+
+// LDL^T without pivoting; returns false when a pivot collapses (rank
+// deficiency), in which case the pseudo-inverse path must solve.
+__device__ __forceinline__ bool ldl_solve(const double N[21], const double c[6], double m[6])
+{
+    double L[6][6];
+    double d[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double s = N[tri(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) s = s - (L[j][k] * L[j][k]) * d[k];
+        if (!(s > 1e-9 * N[tri(j, j)])) return false;
+        d[j] = s;
+#pragma unroll
+        for (int i = j + 1; i < 6; ++i) {
+            double t = N[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) t = t - (L[i][k] * L[j][k]) * d[k];
+            L[i][j] = t / s;
+        }
+    }
+    double y[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double t = c[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) t = t - L[i][k] * y[k];
+        y[i] = t;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y[i] = y[i] / d[i];
+#pragma unroll
+    for (int i = 5; i >= 0; --i) {
+        double t = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 6; ++k) t = t - L[k][i] * m[k];
+        m[i] = t;
+    }
+    return true;
+}
+
+// Cyclic Jacobi on a symmetric n x n matrix (double); eigenvalues end on the
+// diagonal of a, eigenvectors in the columns of v.
+template <int n>
+__device__ void jacobi_sym(double* a, double* v, int sweeps)
+{
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) v[i * n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        int any = 0;
+        for (int p = 0; p < n - 1; ++p) {
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a[p * n + q];
+                if (apq == 0.0) continue;
+                any = 1;
+                double app = a[p * n + p], aqq = a[q * n + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double t;
+                if (fabs(theta) > 1e150) t = 0.5 / theta;
+                else {
+                    double den = fabs(theta) + sqrt(theta * theta + 1.0);
+                    t = 1.0 / den;
+                    if (theta < 0.0) t = -t;
+                }
+                double cs = 1.0 / sqrt(t * t + 1.0);
+                double sn = t * cs;
+                for (int k = 0; k < n; ++k) {
+                    double akp = a[k * n + p], akq = a[k * n + q];
+                    a[k * n + p] = cs * akp - sn * akq;
+                    a[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = a[p * n + k], aqk = a[q * n + k];
+                    a[p * n + k] = cs * apk - sn * aqk;
+                    a[q * n + k] = sn * apk + cs * aqk;
+                }
+                a[p * n + q] = 0.0;
+                a[q * n + p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    double vkp = v[k * n + p], vkq = v[k * n + q];
+                    v[k * n + p] = cs * vkp - sn * vkq;
+                    v[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+        }
+        if (!any) break;
+    }
+}
+
+__device__ __forceinline__ int psd3(float m0, float m1, float m2, float m3, float m4, float m5)
+{
+    double a = m0, b = m1, c = m2, d = m3, e = m4, f = m5;
+    if (a < 0.0 || d < 0.0 || f < 0.0) return 0;
+    if (a * d - b * b < 0.0) return 0;
+    if (a * f - c * c < 0.0) return 0;
+    if (d * f - e * e < 0.0) return 0;
+    double det = a * (d * f - e * e) - b * (b * f - e * c) + c * (b * e - d * c);
+    if (det < 0.0) return 0;
+    return 1;
+}
+
+__device__ __forceinline__ void store_field(float4* field, size_t v, const double m[6])
+{
+    float f[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) f[i] = (float)m[i];
+    int p = psd3(f[0], f[1], f[2], f[3], f[4], f[5]);
+    field[2 * v] = make_float4(f[0], f[1], f[2], f[3]);
+    field[2 * v + 1] = make_float4(f[4], f[5], p ? 1.0f : 0.0f, 0.0f);
+}
+
+// Kernel (a): blur + flip + LAB + metric for one voxel per thread.  A voxel
+// whose normal equations are rank deficient is queued for k_fixup.
+__global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint8_t* __restrict__ rgb,
+                                                 int prepared, const float2* __restrict__ fwd,
+                                                 const float2* __restrict__ bwd, const float* __restrict__ lin_lut,
+                                                 uint32_t* __restrict__ lab, float4* __restrict__ field,
+                                                 int* __restrict__ fix_list, int* __restrict__ fix_count)
+{
+    __shared__ float lin[256];
+    int tid = threadIdx.y * 64 + threadIdx.x;
+    lin[tid] = lin_lut[tid];
+    __syncthreads();
+
+    int x = blockIdx.x * 64 + threadIdx.x;
+    int y = blockIdx.y * 4 + threadIdx.y;
+    int t = blockIdx.z;
+    if (x >= W || y >= H) return;
+    size_t v = ((size_t)t * H + y) * W + x;
+
+    uint32_t cr, cg, cb;
+    if (prepared) {
+        const uint8_t* p = rgb + v * 3;
+        cr = p[0]; cg = p[1]; cb = p[2];
+    } else {
+        // GaussianBlur 3x3 [1,2,1]^2/16, reflect-101, round half up, on the
+        // image row that becomes GL row y after the vertical flip.
+        const uint8_t* fr = rgb + (size_t)t * H * W * 3;
+        int yi = H - 1 - y;
+        uint32_t ar = 0, ag = 0, ab = 0;
+#pragma unroll
+        for (int j = -1; j <= 1; ++j) {
+            int yy = reflect101(yi + j, H);
+            uint32_t wj = j == 0 ? 2u : 1u;
+#pragma unroll
+            for (int i = -1; i <= 1; ++i) {
+                int xx = reflect101(x + i, W);
+                uint32_t w = wj * (i == 0 ? 2u : 1u);
+                const uint8_t* p = fr + ((size_t)yy * W + xx) * 3;
+                ar += w * p[0]; ag += w * p[1]; ab += w * p[2];
+            }
+        }
+        cr = (ar + 8u) >> 4; cg = (ag + 8u) >> 4; cb = (ab + 8u) >> 4;
+    }
+    lab[v] = lab_pack(cr, cg, cb, lin);
+
+    double N[21], c[6], m[6];
+    int rows = normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
+    if (rows == 0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) m[i] = 0.0;
+        store_field(field, v, m);
+        return;
+    }
+    if (ldl_solve(N, c, m)) {
+        store_field(field, v, m);
+    } else {
+        int slot = atomicAdd(fix_count, 1);
+        fix_list[slot] = (int)v;
+    }
+}
+
+// Rank-deficient voxels: minimum-norm solution through the eigen-
+// decomposition of N (pseudo-inverse, eigenvalues below 1e-9 of the largest
+// dropped).
+__global__ __launch_bounds__(64) void k_fixup(int W, int H, int D, const float2* __restrict__ fwd,
+                                              const float2* __restrict__ bwd, float4* __restrict__ field,
+                                              const int* __restrict__ fix_list, const int* __restrict__ fix_count)
+{
+    int n = *fix_count;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        int v = fix_list[idx];
+        int x = v % W;
+        int y = (v / W) % H;
+        int t = v / (W * H);
+        double N[21], c[6];
+        normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
+        double a[36], ev[36];
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) a[i * 6 + j] = N[tri(i, j)];
+        jacobi_sym<6>(a, ev, 30);
+        double lmax = 0.0;
+        for (int i = 0; i < 6; ++i) lmax = a[i * 6 + i] > lmax ? a[i * 6 + i] : lmax;
+        double m[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = 0; e < 6; ++e) {
+            double lam = a[e * 6 + e];
+            if (!(lam > 1e-9 * lmax)) continue;
+            double proj = 0.0;
+            for (int k = 0; k < 6; ++k) proj = proj + ev[k * 6 + e] * c[k];
+            double coef = proj / lam;
+            for (int k = 0; k < 6; ++k) m[k] = m[k] + ev[k * 6 + e] * coef;
+        }
+        store_field(field, (size_t)v, m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// seeds
+// ---------------------------------------------------------------------------
+
+// Iteration 1: grid positions, colour sampled at the seed voxel.
+__global__ void k_seed_init(int W, int H, int A, const int4* __restrict__ grid, const uint32_t* __restrict__ lab,
+                            int4* __restrict__ pos, float4* __restrict__ slab)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A) return;
+    int4 g = grid[i];
+    uint32_t pk = lab[((size_t)g.z * H + g.y) * W + g.x];
+    pos[i] = g;
+    slab[i] = make_float4((float)(pk & 255u), (float)((pk >> 8) & 255u), (float)((pk >> 16) & 255u), 0.0f);
+}
+
+__device__ __constant__ int c_ring1x[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+__device__ __constant__ int c_ring1y[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+__device__ __constant__ int c_ring2x[16] = {-2, -1, 0, 1, 2, -2, 2, -2, 2, -2, 2, -2, -1, 0, 1, 2};
+__device__ __constant__ int c_ring2y[16] = {-2, -2, -2, -2, -2, -1, -1, 0, 0, 1, 1, 2, 2, 2, 2, 2};
+
+// acos(cos)*180/PI < 90 under a correctly rounded float acos holds exactly
+// when cos exceeds this float (bit pattern 0x328885a3).
+#define ASV_GATE_COS_MIN 1.5893254712295857e-08f
+
+__device__ __forceinline__ bool angle_gate(float su, float sv, float nu, float nv)
+{
+    float num = su * nu + sv * nv;
+    double den = sqrt((double)su * (double)su + (double)sv * (double)sv) *
+                 sqrt((double)nu * (double)nu + (double)nv * (double)nv);
+    float cs = (float)((double)num / den);
+    return cs > ASV_GATE_COS_MIN && cs <= 1.0f;
+}
+
+__device__ __forceinline__ float2 flow_gl(const float2* __restrict__ F, int W, int H, int x, int y, int t)
+{
+    float2 f = F[((size_t)t * H + (H - 1 - y)) * W + x];
+    return make_float2(f.x, -f.y);
+}
+
+__device__ void nearest_psd(const float in[6], float out[6])
+{
+    double a[9] = {in[0], in[1], in[2], in[1], in[3], in[4], in[2], in[4], in[5]};
+    double v[9];
+    jacobi_sym<3>(a, v, 30);
+    double lam[3];
+    for (int i = 0; i < 3; ++i) lam[i] = a[i * 3 + i] < 0.0 ? 0.0 : a[i * 3 + i];
+    double r[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int e = 0; e < 3; ++e) s = s + (v[i * 3 + e] * lam[e]) * v[j * 3 + e];
+            r[i * 3 + j] = s;
+        }
+    out[0] = (float)r[0]; out[1] = (float)r[1]; out[2] = (float)r[2];
+    out[3] = (float)r[4]; out[4] = (float)r[5]; out[5] = (float)r[8];
+}
+
+// Metric of every seed: the field at its voxel if PSD, else the mean of the
+// PSD 1-ring (then 2-ring) neighbours on the same frame, else the nearest
+// PSD matrix; the ring neighbours pass the flow-angle gate when `gate`.
+__global__ void k_seed_metric(int W, int H, int D, int A, const int4* __restrict__ pos,
+                              const float4* __restrict__ slab, const float4* __restrict__ field,
+                              const float2* __restrict__ fwd, int gate, SeedRec* __restrict__ rec,
+                              float* __restrict__ trace_iter)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A) return;
+    int4 p = pos[i];
+    size_t v = ((size_t)p.z * H + p.y) * W + p.x;
+    float4 f0 = field[2 * v], f1 = field[2 * v + 1];
+    float m[6];
+    if (f1.z != 0.0f) {
+        m[0] = f0.x; m[1] = f0.y; m[2] = f0.z; m[3] = f0.w; m[4] = f1.x; m[5] = f1.y;
+    } else {
+        float2 sf = make_float2(0.0f, 0.0f);
+        if (gate && p.z != D - 1) sf = flow_gl(fwd, W, H, p.x, p.y, p.z);
+        float sum[6] = {0, 0, 0, 0, 0, 0};
+        int cnt = 0;
+        for (int ring = 0; ring < 2 && cnt == 0; ++ring) {
+            int nr = ring == 0 ? 8 : 16;
+            for (int t = 0; t < nr; ++t) {
+                int nx = p.x + (ring == 0 ? c_ring1x[t] : c_ring2x[t]);
+                int ny = p.y + (ring == 0 ? c_ring1y[t] : c_ring2y[t]);
+                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                if (gate) {
+                    float2 nf = make_float2(0.0f, 0.0f);
+                    if (p.z != D - 1) nf = flow_gl(fwd, W, H, nx, ny, p.z);
+                    if (!angle_gate(sf.x, sf.y, nf.x, nf.y)) continue;
+                }
+                size_t nvx = ((size_t)p.z * H + ny) * W + nx;
+                float4 g0 = field[2 * nvx], g1 = field[2 * nvx + 1];
+                if (g1.z == 0.0f) continue;
+                sum[0] = sum[0] + g0.x; sum[1] = sum[1] + g0.y; sum[2] = sum[2] + g0.z;
+                sum[3] = sum[3] + g0.w; sum[4] = sum[4] + g1.x; sum[5] = sum[5] + g1.y;
+                ++cnt;
+            }
+        }
+        if (cnt > 0) {
+            for (int k = 0; k < 6; ++k) m[k] = sum[k] / (float)cnt;
+        } else {
+            float self[6] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y};
+            nearest_psd(self, m);
+        }
+    }
+    float4 c = slab[i];
+    SeedRec r;
+    r.a = make_float4((float)p.x, (float)p.y, (float)p.z, m[0]);
+    r.b = make_float4(m[1], m[2], m[3], m[4]);
+    r.c = make_float4(m[5], c.x, c.y, c.z);
+    rec[i] = r;
+    if (trace_iter) {
+        float* tr = trace_iter + (size_t)i * 12;
+        tr[0] = (float)p.x; tr[1] = (float)p.y; tr[2] = (float)p.z;
+        tr[3] = c.x; tr[4] = c.y; tr[5] = c.z;
+        for (int k = 0; k < 6; ++k) tr[6 + k] = m[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scatter
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_fill(int4* __restrict__ p4, int64_t n4, int32_t* __restrict__ tail,
+                                              int64_t ntail, int32_t value)
+{
+    int4 val = make_int4(value, value, value, value);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+        p4[i] = val;
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ntail) tail[g] = value;
+}
+
+__global__ void k_zero_u64(uint64_t* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ void k_zero_i32(int* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Seed scatter: on a voxel collision the higher id wins, as the later
+// primitive does under GL's in-order rasterisation.
+__global__ void k_place(int W, int H, int A, const int4* __restrict__ pos, int32_t* __restrict__ labels)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A) return;
+    int4 p = pos[i];
+    atomicMax(&labels[((size_t)p.z * H + p.y) * W + p.x], i);
+}
+
+// ---------------------------------------------------------------------------
+// JFA pass (flood.gs:53-209)
+// ---------------------------------------------------------------------------
+
+// One voxel per thread, 64 x 4 voxels per block.  The 27 taps at clamped
+// (x +- k, y +- k, z +- k) are read in the reference's scan order; a tap
+// whose id matches one of the last two evaluated ids is skipped (its
+// distance is identical, so neither tie rule can change the winner).
+template <int TIE>
+__global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t* __restrict__ in,
+                                             int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
+                                             const SeedRec* __restrict__ rec, int k, float ratio)
+{
+    int x = blockIdx.x * 64 + threadIdx.x;
+    int y = blockIdx.y * 4 + threadIdx.y;
+    int z = blockIdx.z;
+    if (x >= W || y >= H) return;
+    const size_t plane = (size_t)W * H;
+    size_t v = (size_t)z * plane + (size_t)y * W + x;
+    uint32_t pk = lab[v];
+    float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+
+    int xs[3] = {clampi(x - k, 0, W - 1), x, clampi(x + k, 0, W - 1)};
+    int ys[3] = {clampi(y - k, 0, H - 1), y, clampi(y + k, 0, H - 1)};
+    int zs[3] = {z, clampi(z - k, 0, D - 1), clampi(z + k, 0, D - 1)};
+
+    float best = 100000000.0f;
+    int32_t best_id = -1;
+    int32_t seen0 = -1, seen1 = -1;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int32_t* row = in + (size_t)zs[sl] * plane + (size_t)ys[r] * W;
+#pragma unroll
+            for (int cidx = 0; cidx < 3; ++cidx) {
+                int32_t id = row[xs[cidx]];
+                if (id < 0 || id == seen0 || id == seen1) continue;
+                seen1 = seen0;
+                seen0 = id;
+                SeedRec s = rec[id];
+                float dx = (float)x - s.a.x, dy = (float)y - s.a.y, dz = (float)z - s.a.z;
+                float q1 = (dx * dx * s.a.w + 2.0f * dx * dy * s.b.x) + 2.0f * dx * dz * s.b.y;
+                float q2 = (dy * dy * s.b.z + 2.0f * dy * dz * s.b.w) + dz * dz * s.c.x;
+                float q = q1 + q2;
+                q = q < 0.0f ? 0.0f : q;
+                float c0 = l0 - s.c.y, c1 = l1 - s.c.z, c2 = l2 - s.c.w;
+                float d = ratio * q + ((c0 * c0 + c1 * c1) + c2 * c2);
+                bool take = TIE == 0 ? (d < best || (d == best && id < best_id)) : (d < best);
+                if (take) { best = d; best_id = id; }
+            }
+        }
+    }
+    out[v] = best_id;
+}
+
+// ---------------------------------------------------------------------------
+// centroid reduction (centroids.vs:19-53)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One voxel per lane over the linear volume.  Lanes of a wave that share a
+// slot are summed with cross-lane adds and the wave's first such lane issues
+// one 64-bit atomic per sum, so the totals are exact and order-free.
+__global__ __launch_bounds__(256) void k_reduce(int W, int H, int64_t V, int A, const int32_t* __restrict__ labels,
+                                                const uint32_t* __restrict__ lab, unsigned long long* __restrict__ sums)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = v < V;
+    int slot = -1;
+    int vx = 0, vy = 0, vz = 0, vL = 0, va = 0, vb = 0;
+    if (valid) {
+        int l = labels[v];
+        slot = l < 0 ? A : l;
+        int x = (int)(v % W);
+        int64_t r = v / W;
+        int y = (int)(r % H);
+        int z = (int)(r / H);
+        uint32_t pk = lab[v];
+        vx = 2 * x + 1; vy = 2 * y + 1; vz = 2 * z + 1;
+        vL = pk & 255u; va = (pk >> 8) & 255u; vb = (pk >> 16) & 255u;
+    }
+    int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+        int leader = __ffsll((long long)pending) - 1;
+        int s = __shfl(slot, leader, 64);
+        bool mine = valid && slot == s;
+        unsigned long long mask = __ballot(mine);
+        int cx = wave_sum(mine ? vx : 0);
+        int cy = wave_sum(mine ? vy : 0);
+        int cz = wave_sum(mine ? vz : 0);
+        int cL = wave_sum(mine ? vL : 0);
+        int ca = wave_sum(mine ? va : 0);
+        int cb = wave_sum(mine ? vb : 0);
+        if (lane == leader) {
+            unsigned long long* o = sums + (size_t)s * kSumWords;
+            atomicAdd(o + 0, (unsigned long long)cx);
+            atomicAdd(o + 1, (unsigned long long)cy);
+            atomicAdd(o + 2, (unsigned long long)cz);
+            atomicAdd(o + 3, (unsigned long long)__popcll(mask));
+            atomicAdd(o + 4, (unsigned long long)cL);
+            atomicAdd(o + 5, (unsigned long long)ca);
+            atomicAdd(o + 6, (unsigned long long)cb);
+        }
+        pending &= ~mask;
+        if (mine) valid = false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// centroid finalize (aver_exception.vs:22-81)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float slot_mean(const unsigned long long* sums, int A, int j, int axis)
+{
+    if (j > A) return 0.0f;
+    const unsigned long long* s = sums + (size_t)j * kSumWords;
+    unsigned long long n = s[3];
+    if (n == 0ull) return 0.0f;
+    return (float)((double)s[axis] / (2.0 * (double)n));
+}
+
+__global__ void k_finalize(int W, int H, int D, int A, const unsigned long long* __restrict__ sums,
+                           unsigned long long* __restrict__ zero_next, const uint32_t* __restrict__ lab,
+                           int4* __restrict__ pos, float4* __restrict__ slab)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= A) {
+#pragma unroll
+        for (int w = 0; w < kSumWords; ++w) zero_next[(size_t)i * kSumWords + w] = 0ull;
+    }
+    if (i >= A) return;
+    const unsigned long long* s = sums + (size_t)i * kSumWords;
+    unsigned long long n = s[3];
+    int p[3];
+    float4 c;
+    if (n == 0ull) {
+        int jm = i == 0 ? W - 1 : i - 1;
+        int jp = i + 1;
+        int dims[3] = {W, H, D};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float cf = floorf(0.5f * (slot_mean(sums, A, jm, a) + slot_mean(sums, A, jp, a)));
+            p[a] = clampi((int)cf, 0, dims[a] - 1);
+        }
+        uint32_t pk = lab[((size_t)p[2] * H + p[1]) * W + p[0]];
+        c = make_float4((float)(pk & 255u), (float)((pk >> 8) & 255u), (float)((pk >> 16) & 255u), 0.0f);
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] = (int)(s[a] / (2ull * n));
+        c = make_float4((float)((double)s[4] / (double)n), (float)((double)s[5] / (double)n),
+                        (float)((double)s[6] / (double)n), 0.0f);
+    }
+    pos[i] = make_int4(p[0], p[1], p[2], 0);
+    slab[i] = c;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared,
+                          const float2* fwd, const float2* bwd, const float* lin_lut,
+                          uint32_t* lab, float4* field, int* fix_list, int* fix_count)
+{
+    dim3 block(64, 4, 1);
+    dim3 grid(cdiv(d.W, 64), cdiv(d.H, 4), (unsigned)d.D);
+    hipLaunchKernelGGL(k_prepare, grid, block, 0, st, d.W, d.H, d.D, rgb, prepared, fwd, bwd, lin_lut,
+                       lab, field, fix_list, fix_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd,
+                        float4* field, const int* fix_list, const int* fix_count)
+{
+    hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), 0, st, d.W, d.H, d.D, fwd, bwd, field, fix_list, fix_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, const uint32_t* lab,
+                            int4* pos, float4* slab)
+{
+    hipLaunchKernelGGL(k_seed_init, dim3(cdiv(d.A, 256)), dim3(256), 0, st, d.W, d.H, d.A, grid, lab, pos, slab);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab,
+                              const float4* field, const float2* fwd, int gate, SeedRec* rec,
+                              float* trace_iter)
+{
+    hipLaunchKernelGGL(k_seed_metric, dim3(cdiv(d.A, 64)), dim3(64), 0, st, d.W, d.H, d.D, d.A, pos, slab,
+                       field, fwd, gate, rec, trace_iter);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t st, int32_t* p, int64_t n, int32_t value)
+{
+    int64_t n4 = n / 4;
+    int64_t ntail = n - n4 * 4;
+    unsigned blocks = n4 > 0 ? cdiv(n4, 256) : 1;
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, reinterpret_cast<int4*>(p), n4, p + n4 * 4,
+                       ntail, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n)
+{
+    unsigned blocks = n > 0 ? cdiv(n, 256) : 1;
+    if (blocks > 4096u) blocks = 4096u;
+    hipLaunchKernelGGL(k_zero_u64, dim3(blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n)
+{
+    unsigned blocks = n > 0 ? cdiv(n, 256) : 1;
+    if (blocks > 4096u) blocks = 4096u;
+    hipLaunchKernelGGL(k_zero_i32, dim3(blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels)
+{
+    hipLaunchKernelGGL(k_place, dim3(cdiv(d.A, 256)), dim3(256), 0, st, d.W, d.H, d.A, pos, labels);
+    return hipGetLastError();
+}
+
+hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule)
+{
+    dim3 block(64, 4, 1);
+    dim3 grid(cdiv(d.W, 64), cdiv(d.H, 4), (unsigned)d.D);
+    if (tie_rule == 0)
+        hipLaunchKernelGGL(k_jfa<0>, grid, block, 0, st, d.W, d.H, d.D, in, out, lab, rec, k, ratio);
+    else
+        hipLaunchKernelGGL(k_jfa<1>, grid, block, 0, st, d.W, d.H, d.D, in, out, lab, rec, k, ratio);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
+                         unsigned long long* sums)
+{
+    int64_t V = (int64_t)d.W * d.H * d.D;
+    hipLaunchKernelGGL(k_reduce, dim3(cdiv(V, 256)), dim3(256), 0, st, d.W, d.H, V, d.A, labels, lab, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long long* sums,
+                           unsigned long long* zero_next, const uint32_t* lab, int4* pos, float4* slab)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(cdiv(d.A + 1, 64)), dim3(64), 0, st, d.W, d.H, d.D, d.A, sums,
+                       zero_next, lab, pos, slab);
+    return hipGetLastError();
+}
+
+}  // namespace asv

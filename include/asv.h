@@ -1,0 +1,151 @@
+/*
+ * asv.h -- C ABI of the MI355X supervoxel hot path (libasv.so).
+ *
+ * Plain C types only: host pointers, sizes, an opaque context and an opaque
+ * stream handle (a hipStream_t passed as void*, NULL for the null stream).
+ * Every entry point returns an int status (0 = ASV_OK, negative = error) and
+ * never throws; asv_last_error() gives the message of the calling thread's
+ * last failure.  One context serves one clip (or one stream window) on the
+ * current HIP device; a context must not be used by two threads at once.
+ *
+ * The reference drives the same stages through OpenGL from class Image
+ * (GPU-AS/Image.h:71-88), called in order by Render::run
+ * (GPU-AS/Render.cpp:254-319).  Each entry point names the reference method
+ * it replaces.
+ */
+#ifndef ASV_H
+#define ASV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ASV_OK = 0,
+    ASV_ERR_ARG = -1,      /* bad argument or size mismatch */
+    ASV_ERR_HIP = -2,      /* a HIP runtime call failed */
+    ASV_ERR_STATE = -3,    /* call out of order (e.g. run before upload) */
+    ASV_ERR_NOMEM = -4,    /* device allocation failed */
+    ASV_ERR_ADMIT = -5     /* serving: footprint exceeds the memory budget */
+};
+
+enum { ASV_TIE_LOWEST_ID = 0, ASV_TIE_SCAN_ORDER = 1 };
+enum { ASV_SCHED_REFERENCE = 0, ASV_SCHED_HALVING_PLUS_ONE = 1 };
+
+typedef struct asv_params {
+    int32_t width, height, depth;   /* W, H, D (frames) */
+    int32_t seeds_num;              /* K requested; grid places A <= K */
+    int32_t lloyd_iters;            /* Lloyd iterations (reference default 20) */
+    float   compactness;            /* 8 (GPU-AS/Image.cpp:1050) or 6 (streamGPU-AS/Image.cpp:968) */
+    int32_t tie_rule;               /* ASV_TIE_* */
+    int32_t angle_gate;             /* 1: flow-angle gate on ring fallbacks from iteration 2
+                                       (GPU-AS/Image.cpp:1404-1421); 0: none (streamGPU-AS) */
+    int32_t schedule;               /* ASV_SCHED_* */
+    int32_t frames_prepared;        /* 0: raw frames, image row order (row 0 = top);
+                                       1: already blurred and in GL row order */
+} asv_params;
+
+typedef struct asv_ctx asv_ctx;
+
+/* Fills p with the reference's defaults for W x H x D and K seeds. */
+void asv_default_params(asv_params* p, int32_t width, int32_t height, int32_t depth, int32_t seeds_num);
+
+/* Image::init_gpu + Image::init_seeds + init_variables
+ * (GPU-AS/Image.cpp:320-345, :20-158, :1047-1053): validates p, places the
+ * grid seeds and allocates every device buffer of the clip. */
+int asv_create(const asv_params* p, asv_ctx** out);
+int asv_destroy(asv_ctx* ctx);
+
+/* Number of seeds placed (ac_seeds_num, GPU-AS/Image.cpp:45) and grid step s
+ * (GPU-AS/Image.cpp:32). */
+int asv_seeds_placed(const asv_ctx* ctx);
+int asv_grid_step(const asv_ctx* ctx);
+/* Resident device bytes held by the context. */
+int64_t asv_device_bytes(const asv_ctx* ctx);
+/* Device bytes asv_create would allocate for p (admission control). */
+int64_t asv_footprint(const asv_params* p);
+
+/* JFA pass list (GPU-AS/Image.cpp:1061-1181): writes up to cap step lengths
+ * and ratio multipliers, returns the number of passes P. */
+int asv_schedule(const asv_ctx* ctx, int32_t* steps, float* mult, int32_t cap);
+
+/* Grid seed voxels (A x 3 int32, x y z in GL orientation). */
+int asv_grid_seeds(const asv_ctx* ctx, int32_t* xyz, int32_t cap);
+
+/* Frame volume: D x H x W x 3 uint8 RGB (host or device pointer), replacing
+ * the texture upload in Image::init_image_texture (GPU-AS/Image.cpp:198-219). */
+int asv_upload_frames(asv_ctx* ctx, const uint8_t* rgb, void* stream);
+
+/* Ground-truth optical flow, D x H x W x 2 float32 each, image row order,
+ * (u, v) in pixels per frame with v pointing down; replaces the Farneback
+ * fields of Image::forward_backward_flow (GPU-AS/Image.cpp:643-686).
+ * Frame D-1 of fwd and frame 0 of bwd are ignored, as in the reference. */
+int asv_upload_flow(asv_ctx* ctx, const float* fwd, const float* bwd, void* stream);
+
+/* Image::RGB2LAB (GPU-AS/Image.cpp:296-315) fused with the per-voxel
+ * matrix_slover metric (GPU-AS/Image.cpp:874-1009): builds the LAB volume and
+ * the metric field. */
+int asv_prepare(asv_ctx* ctx, void* stream);
+
+/* Image::Ld_first_init (GPU-AS/Image.cpp:1017-1045) for iteration 1 and
+ * Image::Ld_next_seeds_matrix (GPU-AS/Image.cpp:1300-1575) afterwards: seed
+ * metric selection and scatter into the label volume. */
+int asv_seed_stage(asv_ctx* ctx, int32_t iter, void* stream);
+
+/* Image::flood (GPU-AS/Image.cpp:1061-1190): the P JFA passes. */
+int asv_flood(asv_ctx* ctx, int32_t iter, void* stream);
+
+/* Image::centroids_next_seeds (GPU-AS/Image.cpp:1238-1297): reduction, then
+ * centroid finalisation unless iter is the last iteration. */
+int asv_centroids(asv_ctx* ctx, int32_t iter, void* stream);
+
+/* Whole run: asv_prepare, then for iter = 1..lloyd_iters the three stages
+ * above (GPU-AS/Render.cpp:274-293), all enqueued on stream. */
+int asv_run(asv_ctx* ctx, void* stream);
+/* The Lloyd loop only (asv_prepare must have run on the current inputs). */
+int asv_run_loop(asv_ctx* ctx, void* stream);
+
+/* Final label volume (D x H x W int32, GL row order, -1 = unlabelled),
+ * before any post-processing; replaces the label pass + glGetTexImage of
+ * Image::check_dead_seeds (GPU-AS/Image.cpp:1625-1643).  Synchronises stream. */
+int asv_download_labels(asv_ctx* ctx, int32_t* labels, void* stream);
+
+/* Final reduction slots: (A+1) x 8 int64 = sum(2x+1), sum(2y+1), sum(2z+1),
+ * count, sum L, sum a, sum b, 0; slot A collects label -1.  These replace the
+ * slot texels read back at GPU-AS/Image.cpp:1585-1588. */
+int asv_download_sums(asv_ctx* ctx, int64_t* sums, void* stream);
+
+/* Per-iteration seed records, when enabled before the run:
+ * lloyd_iters x A x 12 float (x, y, z, L, a, b, m0..m5). */
+int asv_enable_trace(asv_ctx* ctx, int32_t on);
+int asv_download_trace(asv_ctx* ctx, float* trace, void* stream);
+
+/* LAB volume (D x H x W packed L | a<<8 | b<<16) and metric field
+ * (D x H x W x 8 float: m0..m5, psd flag, 0), for tests and tools. */
+int asv_download_lab(asv_ctx* ctx, uint32_t* lab, void* stream);
+int asv_download_metric(asv_ctx* ctx, float* m8, void* stream);
+
+/* Timed run: same work as asv_run, with HIP events around every kernel
+ * launch on stream.  ms[0] total, ms[1] prepare, ms[2] sum of JFA passes,
+ * ms[3] sum of reductions, ms[4] seed stages + finalisation; passes receives
+ * the JFA pass count. */
+int asv_run_timed(asv_ctx* ctx, void* stream, float* ms, int32_t* passes);
+
+/* Synthetic video of the benchmark generator: background bilinear gradient,
+ * n_ellipses moving ellipses, Gaussian noise, ground-truth forward and
+ * backward flow and region ids (0 = background, e+1 = ellipse e).  Arrays are
+ * D x H x W (x3 / x2), image row order.  Any output may be NULL. */
+int asv_synth_video(uint64_t seed, int32_t width, int32_t height, int32_t depth,
+                    int32_t n_ellipses, float vmax, float noise_sigma,
+                    uint8_t* rgb, float* fwd, float* bwd, int32_t* region);
+
+const char* asv_last_error(void);
+int asv_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASV_H */

@@ -1,0 +1,114 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: bit-exact for everything (labels, integer sums, seed records, LAB
+volume, metric field) -- the kernels and the oracle implement one arithmetic
+specification.
+"""
+import numpy as np
+import pytest
+
+import anisotropic_supervoxel_amd as asv
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _clip_run(rgb, fwd, bwd, K, iters, trace=True, **kw):
+    D, H, W, _ = rgb.shape
+    p = asv.default_params(W, H, D, K, lloyd_iters=iters, **kw)
+    with asv.Clip(p) as c:
+        if trace:
+            c.enable_trace()
+        c.upload(rgb, fwd, bwd)
+        c.run()
+        out = dict(labels=c.labels(), sums=c.sums(), A=c.A)
+        if trace:
+            out["trace"] = c.trace()
+        out["lab"] = c.lab()
+        out["metric"] = c.metric()
+    return out
+
+
+def _assert_run_equal(got, rgb, fwd, bwd, K, iters, compactness=8.0, tie_rule=0, angle_gate=1,
+                      schedule=0, frames_prepared=0):
+    want_labels, want_sums, want_trace, A = orc.run(rgb, fwd, bwd, K, iters, compactness, tie_rule,
+                                                    angle_gate, schedule, frames_prepared, want_trace=True)
+    assert got["A"] == A
+    tr_diff = np.argwhere(np.any(got["trace"] != want_trace, axis=2))
+    assert tr_diff.size == 0, f"seed records differ first at (iter, seed) {tr_diff[:5].tolist()}"
+    bad = int((got["labels"] != want_labels).sum())
+    assert bad == 0, f"{bad} label mismatches"
+    assert np.array_equal(got["sums"][:, :7], want_sums)
+
+
+def test_lab_exhaustive_all_rgb(have_gpu):
+    """All 2^24 RGB triples, GPU LAB kernel vs oracle, bit-exact."""
+    n = 1 << 24
+    idx = np.arange(n, dtype=np.uint32)
+    rgb = np.stack([(idx >> 16) & 255, (idx >> 8) & 255, idx & 255], axis=-1).astype(np.uint8)
+    rgb = rgb.reshape(1, 4096, 4096, 3)
+    fwd = np.zeros((1, 4096, 4096, 2), np.float32)
+    p = asv.default_params(4096, 4096, 1, 1, lloyd_iters=1, frames_prepared=1)
+    with asv.Clip(p) as c:
+        c.upload(rgb, fwd, fwd)
+        c.prepare()
+        got = c.lab()
+    want = orc.lab_volume(rgb)
+    bad = int((got != want).sum())
+    assert bad == 0, f"{bad} of 2^24 LAB values differ"
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_metric_field_random_flows(have_gpu, seed):
+    """Per-voxel metric (LSQ + PSD flag) on flows with discontinuities,
+    including rank-deficient corners, bit-exact."""
+    rng = np.random.default_rng(seed)
+    D, H, W = 6, 29, 37
+    fwd = rng.uniform(-3, 3, (D, H, W, 2)).astype(np.float32)
+    bwd = rng.uniform(-3, 3, (D, H, W, 2)).astype(np.float32)
+    # blocks of exactly-integer flow and zero flow
+    fwd[:, :10, :12] = np.round(fwd[:, :10, :12])
+    bwd[:, 15:, 20:] = 0.0
+    rgb = rng.integers(0, 256, (D, H, W, 3), dtype=np.uint8)
+    p = asv.default_params(W, H, D, 20, lloyd_iters=1)
+    with asv.Clip(p) as c:
+        c.upload(rgb, fwd, bwd)
+        c.prepare()
+        m8 = c.metric()
+    m6, psd = orc.metric_field(fwd, bwd)
+    assert np.array_equal(m8[..., :6].view(np.uint32), m6.view(np.uint32))
+    assert np.array_equal((m8[..., 6] != 0).astype(np.uint8), psd)
+    assert (psd == 0).any(), "case must exercise the non-PSD fallbacks"
+
+
+@pytest.mark.parametrize("seed,shape,K,iters,kw", [
+    (3, (10, 24, 32), 30, 4, {}),
+    (4, (9, 31, 23), 25, 5, {"tie_rule": 1}),
+    (5, (12, 20, 40), 40, 4, {"angle_gate": 0, "compactness": 6.0}),
+    (6, (8, 26, 30), 20, 3, {"frames_prepared": 1}),
+    (7, (11, 22, 35), 30, 3, {"schedule": 1}),
+])
+def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
+    D, H, W = shape
+    rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 6, 2.0, 4.0)
+    got = _clip_run(rgb, fwd, bwd, K, iters, **kw)
+    _assert_run_equal(got, rgb, fwd, bwd, K, iters, **{
+        "compactness": kw.get("compactness", 8.0), "tie_rule": kw.get("tie_rule", 0),
+        "angle_gate": kw.get("angle_gate", 1), "schedule": kw.get("schedule", 0),
+        "frames_prepared": kw.get("frames_prepared", 0)})
+
+
+def test_config1_bit_exact(have_gpu):
+    """160x120x30, seed 0, 12 ellipses, K=500 (A=468), 10 iterations."""
+    rgb, fwd, bwd, _ = asv.synth_video(0, 160, 120, 30, 12, 2.0, 4.0)
+    got = _clip_run(rgb, fwd, bwd, 500, 10)
+    assert got["A"] == 468
+    _assert_run_equal(got, rgb, fwd, bwd, 500, 10)
+
+
+def test_config2_bit_exact(have_gpu):
+    """320x240x85, seed 1, 16 ellipses, K=2000 (A=1955), 10 iterations."""
+    rgb, fwd, bwd, _ = asv.synth_video(1, 320, 240, 85, 16, 2.0, 4.0)
+    got = _clip_run(rgb, fwd, bwd, 2000, 10)
+    assert got["A"] == 1955
+    _assert_run_equal(got, rgb, fwd, bwd, 2000, 10)
