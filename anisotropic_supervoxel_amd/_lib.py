@@ -29,6 +29,7 @@ SIGNATURES = {
     "asv_device_bytes": (_i64, [_vp]),
     "asv_footprint": (_i64, [_vp]),
     "asv_schedule": (ctypes.c_int, [_vp, _vp, _vp, _i32]),
+    "asv_plan": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i32]),
     "asv_grid_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
     "asv_upload_frames": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_upload_flow": (ctypes.c_int, [_vp, _vp, _vp, _vp]),

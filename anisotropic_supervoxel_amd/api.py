@@ -36,13 +36,21 @@ def default_params(width, height, depth, seeds_num, **kw):
     return p
 
 
+def plan(params):
+    """Host-only plan for params: (A seeds placed, grid step s, steps, mult)."""
+    A = ctypes.c_int32(0)
+    s = ctypes.c_int32(0)
+    steps = np.zeros(64, np.int32)
+    mult = np.zeros(64, np.float32)
+    n = _lib.load().asv_plan(ctypes.byref(params), ctypes.byref(A), ctypes.byref(s), ptr(steps), ptr(mult), 64)
+    if n < 0:
+        check(n)
+    return A.value, s.value, steps[:n].tolist(), mult[:n].tolist()
+
+
 def seeds_placed_for(width, height, depth, seeds_num):
     """Grid seeds placed for these sizes (host-side rule, no GPU needed)."""
-    p = default_params(width, height, depth, seeds_num)
-    b = _lib.load().asv_footprint(ctypes.byref(p))
-    if b < 0:
-        raise AsvError("invalid parameters")
-    return p
+    return plan(default_params(width, height, depth, seeds_num))[0]
 
 
 def synth_video(seed, width, height, depth, n_ellipses, vmax=2.0, noise_sigma=4.0):

@@ -245,6 +245,26 @@ int64_t asv_footprint(const asv_params* p)
     return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false);
 }
 
+int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step, int32_t* steps, float* mult,
+             int32_t cap)
+{
+    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    int s = 0;
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, nullptr, &s);
+    if (A <= 0) return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes");
+    if (s < 1) s = 1;
+    std::vector<int> st;
+    std::vector<float> mu;
+    schedule_list(p->schedule, s, p->width, st, mu);
+    if (seeds_placed) *seeds_placed = A;
+    if (grid_step) *grid_step = s;
+    for (size_t i = 0; i < st.size() && (int)i < cap; ++i) {
+        if (steps) steps[i] = st[i];
+        if (mult) mult[i] = mu[i];
+    }
+    return (int)st.size();
+}
+
 int asv_create(const asv_params* p, asv_ctx** out)
 {
     if (!out) return fail(ASV_ERR_ARG, "null output pointer");

@@ -450,7 +450,7 @@ __global__ void k_seed_metric(int W, int H, int D, int A, const int4* __restrict
     float4 c = slab[i];
     SeedRec r;
     r.a = make_float4((float)p.x, (float)p.y, (float)p.z, m[0]);
-    r.b = make_float4(m[1], m[2], m[3], m[4]);
+    r.b = make_float4(2.0f * m[1], 2.0f * m[2], m[3], 2.0f * m[4]);
     r.c = make_float4(m[5], c.x, c.y, c.z);
     rec[i] = r;
     if (trace_iter) {
@@ -501,10 +501,39 @@ __global__ void k_place(int W, int H, int A, const int4* __restrict__ pos, int32
 // JFA pass (flood.gs:53-209)
 // ---------------------------------------------------------------------------
 
-// One voxel per thread, 64 x 4 voxels per block.  The 27 taps at clamped
-// (x +- k, y +- k, z +- k) are read in the reference's scan order; a tap
-// whose id matches one of the last two evaluated ids is skipped (its
-// distance is identical, so neither tie rule can change the winner).
+// Distance of voxel (fx, fy, fz) with colour (l0, l1, l2) to one seed record,
+// in the fixed fused order of the specification:
+//   t1 = fma(dz, 2m2, fma(dy, 2m1, dx*m0)),  t2 = fma(dz, 2m4, dy*m3)
+//   q  = max(0, fma(dx, t1, fma(dy, t2, (dz*dz)*m5)))
+//   d  = fma(ratio, q, fma(c2, c2, fma(c1, c1, c0*c0)))
+__device__ __forceinline__ float cand_dist(const float4 a, const float4 b, const float4 c, float fx, float fy,
+                                           float fz, float l0, float l1, float l2, float ratio)
+{
+    float dx = fx - a.x, dy = fy - a.y, dz = fz - a.z;
+    float t1 = __fmaf_rn(dz, b.y, __fmaf_rn(dy, b.x, dx * a.w));
+    float t2 = __fmaf_rn(dz, b.w, dy * b.z);
+    float q = __fmaf_rn(dx, t1, __fmaf_rn(dy, t2, (dz * dz) * c.x));
+    q = q < 0.0f ? 0.0f : q;
+    float c0 = l0 - c.y, c1 = l1 - c.z, c2 = l2 - c.w;
+    float cd = __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0));
+    return __fmaf_rn(ratio, q, cd);
+}
+
+// Selection key for the lowest-id rule: distances are >= +0 (or NaN), so the
+// float bits order like the values; (d, id) lexicographic minimum = u64 min.
+// The initial key (1e8, 0) rejects d >= 1e8 exactly as `d < min_dist` with
+// min_dist = 1e8 does (flood.gs:90, :122).
+__device__ __forceinline__ unsigned long long dkey(float d, int id)
+{
+    return ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)id;
+}
+#define ASV_KEY_INIT (((unsigned long long)0x4CBEBC20u) << 32)   /* bits of 1e8f */
+
+// Generic pass: one voxel per thread, 64 x 4 voxels per block, taps read
+// from global memory in the reference's scan order (slab z, z-k, z+k; rows
+// y-k, y, y+k; columns x-k, x, x+k).  A tap repeating one of the last two
+// evaluated ids is skipped: its distance is identical, so neither tie rule
+// can change the winner.  Serves any step and the scan-order tie rule.
 template <int TIE>
 __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t* __restrict__ in,
                                              int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
@@ -518,12 +547,14 @@ __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t*
     size_t v = (size_t)z * plane + (size_t)y * W + x;
     uint32_t pk = lab[v];
     float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+    float fx = (float)x, fy = (float)y, fz = (float)z;
 
     int xs[3] = {clampi(x - k, 0, W - 1), x, clampi(x + k, 0, W - 1)};
     int ys[3] = {clampi(y - k, 0, H - 1), y, clampi(y + k, 0, H - 1)};
     int zs[3] = {z, clampi(z - k, 0, D - 1), clampi(z + k, 0, D - 1)};
 
-    float best = 100000000.0f;
+    unsigned long long best = ASV_KEY_INIT;
+    float bestd = 100000000.0f;
     int32_t best_id = -1;
     int32_t seen0 = -1, seen1 = -1;
 #pragma unroll
@@ -538,24 +569,323 @@ __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t*
                 seen1 = seen0;
                 seen0 = id;
                 SeedRec s = rec[id];
-                float dx = (float)x - s.a.x, dy = (float)y - s.a.y, dz = (float)z - s.a.z;
-                float q1 = (dx * dx * s.a.w + 2.0f * dx * dy * s.b.x) + 2.0f * dx * dz * s.b.y;
-                float q2 = (dy * dy * s.b.z + 2.0f * dy * dz * s.b.w) + dz * dz * s.c.x;
-                float q = q1 + q2;
-                q = q < 0.0f ? 0.0f : q;
-                float c0 = l0 - s.c.y, c1 = l1 - s.c.z, c2 = l2 - s.c.w;
-                float d = ratio * q + ((c0 * c0 + c1 * c1) + c2 * c2);
-                bool take = TIE == 0 ? (d < best || (d == best && id < best_id)) : (d < best);
-                if (take) { best = d; best_id = id; }
+                float d = cand_dist(s.a, s.b, s.c, fx, fy, fz, l0, l1, l2, ratio);
+                if (TIE == 0) {
+                    unsigned long long kk = dkey(d, id);
+                    best = kk < best ? kk : best;
+                } else if (d < bestd) {
+                    bestd = d;
+                    best_id = id;
+                }
             }
         }
     }
+    if (TIE == 0) best_id = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
     out[v] = best_id;
+}
+
+// Tiled pass for the lowest-id rule and step K (template), 16 x 16 voxels of
+// one slice per block, four 8 x 8 waves.
+//  1. The labels of the three slabs (z, z-K, z+K) over the tile grown by K
+//     on each side are staged in LDS with clamped coordinates, so every tap
+//     is a fixed LDS offset and CLAMP_TO_EDGE is already applied.
+//  2. The distinct seed ids of that region are hashed into at most 63
+//     compact indices (LDS open addressing), their records copied to LDS.
+//  3. Each lane ORs the compact index of its 27 taps into a 64-bit mask, then
+//     evaluates each set bit once: exact per-voxel de-duplication, and the
+//     wave runs as many rounds as its busiest lane has distinct candidates.
+// A block whose region holds more than 63 ids takes the per-tap path over
+// the staged labels.  Blocks are mapped so that each XCD (blocks b with the
+// same b % 8) sweeps one contiguous range of tiles (z-major), keeping the
+// slabs it re-reads in its own L2.
+constexpr int kTile = 16;
+constexpr int kHashSlots = 128;
+constexpr int kMaxCand = 63;
+constexpr int kEmpty = -2;
+
+__device__ __forceinline__ unsigned hash_slot(int id) { return ((unsigned)id * 2654435761u) >> 25; }
+
+template <int K>
+__global__ __launch_bounds__(256) void k_jfa_tile(int W, int H, int D, int tiles_x, int tiles_y, int ntiles,
+                                                  const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                  const uint32_t* __restrict__ lab, const SeedRec* __restrict__ rec,
+                                                  float ratio)
+{
+    constexpr int R = kTile + 2 * K;
+    constexpr int RA = R * R;
+    constexpr int NV = 3 * RA;
+    constexpr int CH = (NV + 255) / 256;
+    __shared__ int s_ids[NV];
+    __shared__ uint8_t s_idx[NV];
+    __shared__ int s_hkey[kHashSlots];
+    __shared__ uint8_t s_hidx[kHashSlots];
+    __shared__ float4 s_rec[kMaxCand * 3];
+    __shared__ int s_rid[kMaxCand];
+    __shared__ int s_wcount[2];
+
+    const int tid = threadIdx.x;
+    // XCD-aware tile order: hardware sends block b to XCD b % 8.
+    const int b = blockIdx.x;
+    const int q = ntiles >> 3, rm = ntiles & 7;
+    const int xcd = b & 7;
+    const int tile = xcd * q + min(xcd, rm) + (b >> 3);
+    const int tx = tile % tiles_x;
+    const int ty = (tile / tiles_x) % tiles_y;
+    const int z = tile / (tiles_x * tiles_y);
+    const int x0 = tx * kTile, y0 = ty * kTile;
+
+    if (tid < kHashSlots) s_hkey[tid] = kEmpty;
+
+    // 1. stage the region
+    const int zs0 = z, zs1 = clampi(z - K, 0, D - 1), zs2 = clampi(z + K, 0, D - 1);
+    for (int i = tid; i < NV; i += 256) {
+        int sl = i / RA;
+        int r = i - sl * RA;
+        int ry = r / R, rx = r - ry * R;
+        int gz = sl == 0 ? zs0 : (sl == 1 ? zs1 : zs2);
+        int gy = clampi(y0 - K + ry, 0, H - 1);
+        int gx = clampi(x0 - K + rx, 0, W - 1);
+        s_ids[i] = in[((size_t)gz * H + gy) * W + gx];
+    }
+    __syncthreads();
+
+    // 2. hash the distinct ids; each thread scans a contiguous chunk and
+    //    inserts only where the id changes
+    {
+        int prev = -1;
+        const int beg = tid * CH, end = min(beg + CH, NV);
+        for (int i = beg; i < end; ++i) {
+            int id = s_ids[i];
+            if (id < 0 || id == prev) continue;
+            prev = id;
+            unsigned h = hash_slot(id);
+            for (int p = 0; p < kHashSlots; ++p) {
+                int old = atomicCAS(&s_hkey[h], kEmpty, id);
+                if (old == kEmpty || old == id) break;
+                h = (h + 1) & (kHashSlots - 1);
+            }
+        }
+    }
+    __syncthreads();
+    // compact indices of the occupied slots (waves 0 and 1 cover 128 slots)
+    int occ = 0, rank = 0;
+    if (tid < kHashSlots) {
+        occ = s_hkey[tid] != kEmpty;
+        unsigned long long m = __ballot(occ);
+        rank = __popcll(m & ((1ull << (tid & 63)) - 1ull));
+        if ((tid & 63) == 0) s_wcount[tid >> 6] = __popcll(m);
+    }
+    __syncthreads();
+    const int ncand = s_wcount[0] + s_wcount[1];
+    const bool overflow = ncand > kMaxCand;
+    if (!overflow) {
+        if (tid < kHashSlots) {
+            int ci = rank + ((tid >> 6) ? s_wcount[0] : 0);
+            s_hidx[tid] = occ ? (uint8_t)ci : (uint8_t)kMaxCand;
+            if (occ) {
+                int id = s_hkey[tid];
+                SeedRec r = rec[id];
+                s_rec[ci * 3 + 0] = r.a;
+                s_rec[ci * 3 + 1] = r.b;
+                s_rec[ci * 3 + 2] = r.c;
+                s_rid[ci] = id;
+            }
+        }
+        __syncthreads();
+        // compact index of every staged label (63 = no label)
+        int prev = -3;
+        uint8_t pidx = (uint8_t)kMaxCand;
+        const int beg = tid * CH, end = min(beg + CH, NV);
+        for (int i = beg; i < end; ++i) {
+            int id = s_ids[i];
+            if (id != prev) {
+                prev = id;
+                pidx = (uint8_t)kMaxCand;
+                if (id >= 0) {
+                    unsigned h = hash_slot(id);
+                    for (int p = 0; p < kHashSlots; ++p) {
+                        if (s_hkey[h] == id) { pidx = s_hidx[h]; break; }
+                        h = (h + 1) & (kHashSlots - 1);
+                    }
+                }
+            }
+            s_idx[i] = pidx;
+        }
+    }
+    __syncthreads();
+
+    // 3. per-voxel evaluation
+    const int wave = tid >> 6, lane = tid & 63;
+    const int lx = (wave & 1) * 8 + (lane & 7);
+    const int ly = (wave >> 1) * 8 + (lane >> 3);
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= W || y >= H) return;
+    const size_t v = ((size_t)z * H + y) * W + x;
+    const uint32_t pk = lab[v];
+    const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    const int base = (ly + K) * R + (lx + K);
+    unsigned long long best = ASV_KEY_INIT;
+
+    if (!overflow) {
+        unsigned long long mask = 0ull;
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx)
+                    mask |= 1ull << s_idx[sl * RA + base + dy * K * R + dx * K];
+        mask &= ~(1ull << kMaxCand);
+        while (mask) {
+            int j = __ffsll((long long)mask) - 1;
+            mask &= mask - 1ull;
+            float d = cand_dist(s_rec[j * 3 + 0], s_rec[j * 3 + 1], s_rec[j * 3 + 2], fx, fy, fz, l0, l1, l2, ratio);
+            unsigned long long kk = dkey(d, s_rid[j]);
+            best = kk < best ? kk : best;
+        }
+    } else {
+        int seen0 = -1, seen1 = -1;
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int id = s_ids[sl * RA + base + dy * K * R + dx * K];
+                    if (id < 0 || id == seen0 || id == seen1) continue;
+                    seen1 = seen0;
+                    seen0 = id;
+                    SeedRec s = rec[id];
+                    float d = cand_dist(s.a, s.b, s.c, fx, fy, fz, l0, l1, l2, ratio);
+                    unsigned long long kk = dkey(d, id);
+                    best = kk < best ? kk : best;
+                }
+    }
+    out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
 }
 
 // ---------------------------------------------------------------------------
 // centroid reduction (centroids.vs:19-53)
 // ---------------------------------------------------------------------------
+
+// Each thread walks 16 consecutive voxels of the linear volume, keeping a
+// run of equal labels in registers; for a run of n voxels from x = a to
+// x = e on row (y, z) the position sums are closed forms:
+//   sum(2x+1) = (e+1)^2 - a^2,  sum(2y+1) = n(2y+1),  sum(2z+1) = n(2z+1),
+// and only L, a, b are accumulated.  A finished run is added into the
+// block's LDS table (open addressing by label, 64-bit LDS atomics); the
+// table is flushed to the global slots with one 64-bit atomic per sum and
+// label at the end.  Integer sums are exact, so the result does not depend
+// on the order of any of these additions.
+constexpr int kSeg = 16;                 // voxels per thread per chunk
+constexpr int kRedChunks = 4;            // chunks per block: 256*16*4 voxels
+constexpr int kRedSlots = 256;
+
+__device__ __forceinline__ void run_flush(int label, int A, int xa, int xe, int y, int z, unsigned sl,
+                                          unsigned sa, unsigned sb, int* s_key,
+                                          unsigned long long (*s_acc)[8], unsigned long long* __restrict__ sums)
+{
+    int key = label < 0 ? A : label;
+    unsigned long long n = (unsigned long long)(xe - xa + 1);
+    unsigned long long vals[7];
+    vals[0] = (unsigned long long)(xe + 1) * (unsigned long long)(xe + 1) - (unsigned long long)xa * (unsigned long long)xa;
+    vals[1] = n * (unsigned long long)(2 * y + 1);
+    vals[2] = n * (unsigned long long)(2 * z + 1);
+    vals[3] = n;
+    vals[4] = sl;
+    vals[5] = sa;
+    vals[6] = sb;
+    unsigned h = hash_slot(key) << 1;           // 8-bit start slot
+    h &= kRedSlots - 1;
+    for (int p = 0; p < kRedSlots; ++p) {
+        int old = atomicCAS(&s_key[h], kEmpty, key);
+        if (old == kEmpty || old == key) {
+#pragma unroll
+            for (int i = 0; i < 7; ++i) atomicAdd(&s_acc[h][i], vals[i]);
+            return;
+        }
+        h = (h + 1) & (kRedSlots - 1);
+    }
+    // table full: straight to the global slot
+    unsigned long long* o = sums + (size_t)key * kSumWords;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) atomicAdd(o + i, vals[i]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_runs(int W, int H, int64_t V, int A,
+                                                     const int32_t* __restrict__ labels,
+                                                     const uint32_t* __restrict__ lab,
+                                                     unsigned long long* __restrict__ sums)
+{
+    __shared__ int s_key[kRedSlots];
+    __shared__ unsigned long long s_acc[kRedSlots][8];
+    const int tid = threadIdx.x;
+    s_key[tid] = kEmpty;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s_acc[tid][i] = 0ull;
+    __syncthreads();
+
+    const int64_t block_base = (int64_t)blockIdx.x * (256 * kSeg * kRedChunks);
+    for (int ch = 0; ch < kRedChunks; ++ch) {
+        const int64_t v0 = block_base + ((int64_t)ch * 256 + tid) * kSeg;
+        if (v0 >= V) break;
+        int lv[kSeg];
+        uint32_t cv[kSeg];
+        if (v0 + kSeg <= V) {
+            const int4* lp = reinterpret_cast<const int4*>(labels + v0);
+            const uint4* cp = reinterpret_cast<const uint4*>(lab + v0);
+#pragma unroll
+            for (int q = 0; q < kSeg / 4; ++q) {
+                int4 a = lp[q];
+                uint4 c = cp[q];
+                lv[4 * q] = a.x; lv[4 * q + 1] = a.y; lv[4 * q + 2] = a.z; lv[4 * q + 3] = a.w;
+                cv[4 * q] = c.x; cv[4 * q + 1] = c.y; cv[4 * q + 2] = c.z; cv[4 * q + 3] = c.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < kSeg; ++q) {
+                bool ok = v0 + q < V;
+                lv[q] = ok ? labels[v0 + q] : 0;
+                cv[q] = ok ? lab[v0 + q] : 0u;
+            }
+        }
+        const int nval = (int)min((int64_t)kSeg, V - v0);
+        int x = (int)(v0 % W);
+        int64_t r = v0 / W;
+        int y = (int)(r % H);
+        int z = (int)(r / H);
+        int cur = lv[0];
+        int xa = x;
+        unsigned sl = cv[0] & 255u, sa = (cv[0] >> 8) & 255u, sb = (cv[0] >> 16) & 255u;
+#pragma unroll
+        for (int q = 1; q < kSeg; ++q) {
+            if (q < nval) {
+                int nx = x + 1, ny = y, nz = z;
+                if (nx == W) { nx = 0; ++ny; if (ny == H) { ny = 0; ++nz; } }
+                int l = lv[q];
+                if (l != cur || nx == 0) {
+                    run_flush(cur, A, xa, x, y, z, sl, sa, sb, s_key, s_acc, sums);
+                    cur = l;
+                    xa = nx;
+                    sl = 0u; sa = 0u; sb = 0u;
+                }
+                uint32_t c = cv[q];
+                sl += c & 255u; sa += (c >> 8) & 255u; sb += (c >> 16) & 255u;
+                x = nx; y = ny; z = nz;
+            }
+        }
+        run_flush(cur, A, xa, x, y, z, sl, sa, sb, s_key, s_acc, sums);
+    }
+    __syncthreads();
+    int key = s_key[tid];
+    if (key != kEmpty) {
+        unsigned long long* o = sums + (size_t)key * kSumWords;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) atomicAdd(o + i, s_acc[tid][i]);
+    }
+}
+
 
 __device__ __forceinline__ int wave_sum(int v)
 {
@@ -734,9 +1064,41 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
     return hipGetLastError();
 }
 
+template <int K>
+static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                        const SeedRec* rec, float ratio)
+{
+    int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
+    int nt = tx * ty * d.D;
+    hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
+                       rec, ratio);
+}
+
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule)
 {
+    int64_t ntiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile) * d.D;
+    if (tie_rule == 0 && ntiles < (1ll << 31)) {
+        switch (k) {
+        case 1: launch_tile<1>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 2: launch_tile<2>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 3: launch_tile<3>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 4: launch_tile<4>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 5: launch_tile<5>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 6: launch_tile<6>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 7: launch_tile<7>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 8: launch_tile<8>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 9: launch_tile<9>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 10: launch_tile<10>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 11: launch_tile<11>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 12: launch_tile<12>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 13: launch_tile<13>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 14: launch_tile<14>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 15: launch_tile<15>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 16: launch_tile<16>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        default: break;
+        }
+    }
     dim3 block(64, 4, 1);
     dim3 grid(cdiv(d.W, 64), cdiv(d.H, 4), (unsigned)d.D);
     if (tie_rule == 0)
@@ -750,7 +1112,9 @@ hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, c
                          unsigned long long* sums)
 {
     int64_t V = (int64_t)d.W * d.H * d.D;
-    hipLaunchKernelGGL(k_reduce, dim3(cdiv(V, 256)), dim3(256), 0, st, d.W, d.H, V, d.A, labels, lab, sums);
+    int64_t per_block = 256 * kSeg * kRedChunks;
+    hipLaunchKernelGGL(k_reduce_runs, dim3(cdiv(V, per_block)), dim3(256), 0, st, d.W, d.H, V, d.A, labels, lab,
+                       sums);
     return hipGetLastError();
 }
 

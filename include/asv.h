@@ -71,6 +71,12 @@ int64_t asv_footprint(const asv_params* p);
  * and ratio multipliers, returns the number of passes P. */
 int asv_schedule(const asv_ctx* ctx, int32_t* steps, float* mult, int32_t cap);
 
+/* Host-only planning (no device needed): seeds placed A, grid step s and the
+ * JFA pass list for p, from the same rules asv_create applies.  Returns the
+ * pass count P, or a negative status. */
+int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step,
+             int32_t* steps, float* mult, int32_t cap);
+
 /* Grid seed voxels (A x 3 int32, x y z in GL orientation). */
 int asv_grid_seeds(const asv_ctx* ctx, int32_t* xyz, int32_t cap);
 

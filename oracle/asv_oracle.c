@@ -8,7 +8,8 @@
  * It restates, in plain C, what the reference GLSL passes and host code do
  * (citations are relative to the reference repository root), under one fixed
  * arithmetic specification that the HIP kernels implement independently:
- *   - no FMA contraction (built with -ffp-contract=off), IEEE float/double
+ *   - no implicit FMA contraction (built with -ffp-contract=off); the JFA
+ *     distance uses explicit fmaf in a fixed order; IEEE float/double
  *     division and square root;
  *   - sRGB linearisation through a 256-entry table (double pow, rounded to
  *     float), cube root by a fixed Halley iteration in double;
@@ -542,17 +543,24 @@ typedef struct {
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-/* Candidate distance, flood.gs:107-121 */
+/* Candidate distance, flood.gs:107-121.  The quadratic form and the colour
+ * distance are evaluated with explicit fused multiply-adds in this order
+ * (the reference leaves contraction to the GL driver; this fixes one):
+ *   t1 = fma(dz, 2m2, fma(dy, 2m1, dx*m0)),  t2 = fma(dz, 2m4, dy*m3)
+ *   q  = max(0, fma(dx, t1, fma(dy, t2, (dz*dz)*m5)))
+ *   cd = fma(c2, c2, fma(c1, c1, c0*c0)),    d = fma(ratio, q, cd)
+ * which is dx(m0dx+2m1dy+2m2dz) + dy(m3dy+2m4dz) + m5dz^2 = delta^T M delta. */
 static inline float cand_dist(const orc_seed* s, int x, int y, int z, const float lab[3], float ratio)
 {
     float dx = (float)(x - s->x), dy = (float)(y - s->y), dz = (float)(z - s->z);
-    float q1 = (dx * dx * s->m[0] + 2.0f * dx * dy * s->m[1]) + 2.0f * dx * dz * s->m[2];
-    float q2 = (dy * dy * s->m[3] + 2.0f * dy * dz * s->m[4]) + dz * dz * s->m[5];
-    float q = q1 + q2;
+    float m1d = 2.0f * s->m[1], m2d = 2.0f * s->m[2], m4d = 2.0f * s->m[4];
+    float t1 = fmaf(dz, m2d, fmaf(dy, m1d, dx * s->m[0]));
+    float t2 = fmaf(dz, m4d, dy * s->m[3]);
+    float q = fmaf(dx, t1, fmaf(dy, t2, (dz * dz) * s->m[5]));
     if (q < 0.0f) q = 0.0f;                                   /* flood.gs:117-120 */
     float c0 = lab[0] - s->lab[0], c1 = lab[1] - s->lab[1], c2 = lab[2] - s->lab[2];
-    float cd = (c0 * c0 + c1 * c1) + c2 * c2;                 /* flood.gs:116 */
-    return ratio * q + cd;                                    /* flood.gs:121 */
+    float cd = fmaf(c2, c2, fmaf(c1, c1, c0 * c0));           /* flood.gs:116 */
+    return fmaf(ratio, q, cd);                                /* flood.gs:121 */
 }
 
 static const int k_off9x[9] = {-1, 0, 1, -1, 0, 1, -1, 0, 1};   /* flood.gs:63-73 */

@@ -180,6 +180,15 @@ def reduce(labels, lab, A):
     return sums
 
 
+def finalize(sums, lab, W, H, D, A):
+    """orc_finalize on (A+1) x 7 sums; returns [(x, y, z, (L, a, b))] * A."""
+    sums = np.ascontiguousarray(sums, np.int64)
+    lab = np.ascontiguousarray(lab, np.uint32)
+    arr = (OrcSeed * A)()
+    load().orc_finalize(_p(sums), _p(lab), W, H, D, A, ctypes.cast(arr, ctypes.c_void_p))
+    return [(arr[i].x, arr[i].y, arr[i].z, tuple(arr[i].lab)) for i in range(A)]
+
+
 def run(rgb, fwd, bwd, K, iters, compactness=8.0, tie_rule=0, angle_gate=1, schedule_mode=0,
         prepared=0, want_trace=False):
     """Whole plain run.  Returns (labels [D,H,W], sums [A+1,7], trace or None, A)."""

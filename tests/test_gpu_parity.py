@@ -87,6 +87,10 @@ def test_metric_field_random_flows(have_gpu, seed):
     (5, (12, 20, 40), 40, 4, {"angle_gate": 0, "compactness": 6.0}),
     (6, (8, 26, 30), 20, 3, {"frames_prepared": 1}),
     (7, (11, 22, 35), 30, 3, {"schedule": 1}),
+    # dense seeds: tile regions hold more than 63 ids (per-tap fallback path)
+    (8, (12, 48, 48), 2000, 3, {}),
+    # partial tiles on both axes, steps above the tiled range (generic kernel)
+    (9, (6, 37, 70), 6, 3, {}),
 ])
 def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
     D, H, W = shape

@@ -1,0 +1,126 @@
+"""CPU tier for the product library and fixtures: libasv.so loads and exports
+every symbol include/asv.h declares; host-only entry points (planning, the
+synthetic generator) agree with the oracle and the committed golden file.
+No GPU compute is called here."""
+import ctypes
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import anisotropic_supervoxel_amd as asv
+from anisotropic_supervoxel_amd import _lib
+import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"\b(asv_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_symbols()
+    assert len(declared) >= 25
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding types every declared entry point
+    assert declared <= set(_lib.SIGNATURES), declared - set(_lib.SIGNATURES)
+
+
+def test_params_defaults_match_reference():
+    p = asv.default_params(160, 120, 30, 500)
+    assert p.lloyd_iters == 20          # GPU-AS/Render.cpp:11
+    assert p.compactness == 8.0         # GPU-AS/Image.cpp:1050
+    assert p.tie_rule == 0 and p.angle_gate == 1 and p.schedule == 0 and p.frames_prepared == 0
+
+
+@pytest.mark.parametrize("W,H,D,K", [(160, 120, 30, 500), (320, 240, 85, 2000), (640, 480, 200, 10000),
+                                     (320, 240, 10, 768), (1280, 720, 10, 3686), (480, 360, 57, 1234),
+                                     (48, 48, 12, 2000), (70, 37, 6, 6)])
+def test_plan_matches_oracle(W, H, D, K):
+    for mode in (0, 1):
+        p = asv.default_params(W, H, D, K, schedule=mode)
+        A, s, steps, mult = asv.plan(p)
+        xyz, s_o = orc.grid_seeds(W, H, D, K)
+        assert A == len(xyz) and s == max(s_o, 1)
+        st_o, mu_o = orc.schedule(mode, s, W)
+        assert steps == st_o and mult == mu_o
+
+
+def test_plan_rejects_bad_params():
+    for kw in [dict(width=1), dict(seeds_num=0), dict(lloyd_iters=0), dict(tie_rule=5), dict(schedule=9)]:
+        p = asv.default_params(32, 32, 4, 10)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        with pytest.raises(asv.api.AsvError, match="ASV_ERR_ARG"):
+            asv.plan(p)
+    assert _lib.load().asv_footprint(ctypes.byref(asv.default_params(1, 1, 1, 1))) < 0
+
+
+def test_footprint_scales_with_volume():
+    f1 = _lib.load().asv_footprint(ctypes.byref(asv.default_params(160, 120, 30, 500)))
+    f3 = _lib.load().asv_footprint(ctypes.byref(asv.default_params(640, 480, 200, 10000)))
+    V1, V3 = 160 * 120 * 30, 640 * 480 * 200
+    assert 60 * V1 < f1 < 80 * V1 and 60 * V3 < f3 < 80 * V3
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_synth_matches_golden_and_is_deterministic():
+    for key, hashes in GOLDEN["synth_sha256"].items():
+        seed, size, E, vm, sg = key.split("-")
+        W, H, D = map(int, size.split("x"))
+        v = asv.synth_video(int(seed), W, H, D, int(E), float(vm), float(sg))
+        assert [_sha(x) for x in v] == hashes, key
+
+
+def test_synth_flow_and_regions_consistent():
+    rgb, fwd, bwd, reg = asv.synth_video(3, 96, 64, 6, 7, 3.0, 0.0)
+    assert np.array_equal(bwd, -fwd)
+    bg = reg == 0
+    assert (fwd[bg] == 0).all()
+    for r in np.unique(reg[~bg]):
+        v = fwd[reg == r]
+        assert (v == v[0]).all() and np.abs(v).max() <= 3.0
+    # zero noise: each ellipse region has one colour
+    for r in np.unique(reg[~bg]):
+        c = rgb[reg == r]
+        assert (c == c[0]).all()
+
+
+def test_golden_lab_samples():
+    for r, g, b, L, A, B in GOLDEN["lab_samples"]:
+        assert orc.lab_of(r, g, b) == (L, A, B)
+
+
+def test_golden_oracle_runs():
+    for key, want in GOLDEN["oracle_runs"].items():
+        seed, size, K, it = key.split("-")
+        W, H, D = map(int, size.split("x"))
+        E = {"40x32x10": 5, "160x120x30": 12}[size]
+        rgb, fwd, bwd, _ = asv.synth_video(int(seed), W, H, D, E, 2.0, 4.0)
+        labels, sums, _, A = orc.run(rgb, fwd, bwd, int(K[1:]), int(it[2:]))
+        assert A == want["A"]
+        assert _sha(labels) == want["labels_sha256"], key
+        assert _sha(sums) == want["sums_sha256"], key
+
+
+def test_gpu_entry_points_fail_loudly_without_device():
+    """Without a GPU, creating a clip must raise (no silent CPU fallback)."""
+    if _lib.load().asv_device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(asv.api.AsvError):
+        asv.Clip(asv.default_params(32, 32, 4, 10))
