@@ -47,6 +47,10 @@ SIGNATURES = {
     "asv_download_metric": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_run_timed": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "asv_synth_video": (ctypes.c_int, [ctypes.c_uint64, _i32, _i32, _i32, _i32, _f32, _f32, _vp, _vp, _vp, _vp]),
+    "asv_postprocess": (ctypes.c_int, [_vp, _i32, _vp, _i64, _vp, _vp]),
+    "asv_write_label_txt": (ctypes.c_int, [_vp, _i32, _i32, _i32, ctypes.c_char_p]),
+    "asv_write_render_ppm": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, ctypes.c_char_p]),
+    "asv_eval_metrics": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "asv_last_error": (ctypes.c_char_p, []),
     "asv_device_count": (ctypes.c_int, []),
 }

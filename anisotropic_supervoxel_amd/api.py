@@ -66,6 +66,42 @@ def synth_video(seed, width, height, depth, n_ellipses, vmax=2.0, noise_sigma=4.
     return rgb, fwd, bwd, reg
 
 
+def postprocess(sums, A, labels):
+    """Image::check_dead_seeds on host arrays (asv_postprocess).  Returns
+    (labels after post-processing, dead seed count, final label count)."""
+    sums8 = np.zeros((A + 1, 8), np.int64)
+    sums8[:, :sums.shape[1]] = sums[:, :8]
+    out = np.ascontiguousarray(labels, np.int32).copy()
+    dead = ctypes.c_int32(0)
+    nfin = ctypes.c_int32(0)
+    check(_lib.load().asv_postprocess(ptr(sums8), A, ptr(out), out.size, ctypes.byref(dead), ctypes.byref(nfin)))
+    return out, dead.value, nfin.value
+
+
+def write_label_txt(labels, path):
+    D, H, W = labels.shape
+    lab = np.ascontiguousarray(labels, np.int32)
+    check(_lib.load().asv_write_label_txt(ptr(lab), W, H, D, str(path).encode()))
+
+
+def write_render_ppm(labels, n_labels, prefix):
+    D, H, W = labels.shape
+    lab = np.ascontiguousarray(labels, np.int32)
+    check(_lib.load().asv_write_render_ppm(ptr(lab), W, H, D, n_labels, str(prefix).encode()))
+
+
+def eval_metrics(labels, regions):
+    """(3D undersegmentation error, 3D boundary recall) of labels against
+    ground-truth regions of the same shape and row order."""
+    D, H, W = labels.shape
+    a = np.ascontiguousarray(labels, np.int32)
+    r = np.ascontiguousarray(regions, np.int32)
+    ue = ctypes.c_double(0.0)
+    br = ctypes.c_double(0.0)
+    check(_lib.load().asv_eval_metrics(ptr(a), ptr(r), W, H, D, ctypes.byref(ue), ctypes.byref(br)))
+    return ue.value, br.value
+
+
 class Clip:
     """One clip's device context (asv_ctx)."""
 

@@ -765,6 +765,213 @@ __global__ __launch_bounds__(256) void k_jfa_tile(int W, int H, int D, int tiles
     out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
 }
 
+// Persistent tiled pass for the lowest-id rule when the seed count fits the
+// LDS id table (A <= kDirectIds).  Per 16 x 16 tile of one slice:
+//  1. each wave loads rows of the three slabs (z, z-K, z+K) over the tile
+//     grown by K, one lane per column, with clamped coordinates (so every
+//     later tap is a fixed offset and CLAMP_TO_EDGE is already applied);
+//  2. ids are claimed in a direct-mapped byte table slot[id] (read first,
+//     32-bit CAS only on an unclaimed byte); the claimer appends the id to
+//     the tile's candidate list;
+//  3. after a barrier the staged ids become 6-bit compact indices and the
+//     candidates' records are copied to LDS;
+//  4. each lane ORs its 27 taps into a 64-bit mask and evaluates each set
+//     bit once (exact de-duplication);
+//  5. the claimed bytes are reset for the next tile.
+// More than 63 candidates in a tile: per-tap path over global labels.
+// Blocks are persistent; XCD x (blocks b with b % 8 == x) sweeps one
+// contiguous z-major range of tiles.
+constexpr int kDirectIds = 16384;
+constexpr int kListCap = 512;
+constexpr unsigned kSlotEmpty = 0xFFu;
+constexpr unsigned kSlotClaimed = 0xFEu;
+
+template <int K>
+__global__ __launch_bounds__(256) void k_jfa_direct(int W, int H, int D, int A, int tiles_x, int tiles_y, int ntiles,
+                                                    const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                    const uint32_t* __restrict__ lab,
+                                                    const SeedRec* __restrict__ rec, float ratio)
+{
+    constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
+    constexpr int RA = R * R;
+    constexpr int RPI = 64 / R;               // staged rows per wave instruction
+    constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
+    __shared__ uint32_t s_slot[kDirectIds / 4];
+    __shared__ uint8_t s_idx[3 * RA];
+    __shared__ float4 s_rec[kMaxCand * 3];
+    __shared__ int s_rid[kMaxCand];
+    __shared__ int s_list[kListCap];
+    __shared__ int s_count;
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int nwords = (A + 3) >> 2;
+    for (int i = tid; i < nwords; i += 256) s_slot[i] = 0xFFFFFFFFu;
+    if (tid == 0) s_count = 0;
+    __syncthreads();
+
+    // persistent XCD-aware tile walk
+    const int nb = gridDim.x;                 // a multiple of 8
+    const int xcd = blockIdx.x & 7, slot_in_xcd = blockIdx.x >> 3, per_xcd = nb >> 3;
+    const int q = ntiles >> 3, rm = ntiles & 7;
+    const int t_beg = xcd * q + min(xcd, rm);
+    const int t_end = t_beg + q + (xcd < rm ? 1 : 0);
+
+    const int sub = lane / R, col = lane - (lane / R) * R;
+    const bool lane_ok = sub < RPI;
+    int ids[3 * NRS];
+    auto stage = [&](int t) {
+        const int sx0 = (t % tiles_x) * kTile, sy0 = ((t / tiles_x) % tiles_y) * kTile;
+        const int sz = t / (tiles_x * tiles_y);
+        const int gx = clampi(sx0 - K + col, 0, W - 1);
+#pragma unroll
+        for (int sl = 0; sl < 3; ++sl) {
+            const int gz = sl == 0 ? sz : clampi(sl == 1 ? sz - K : sz + K, 0, D - 1);
+            const int32_t* slab = in + (size_t)gz * H * W;
+#pragma unroll
+            for (int i = 0; i < NRS; ++i) {
+                const int r = (wave + 4 * i) * RPI + sub;
+                int id = -1;
+                if (lane_ok && r < R) {
+                    const int gy = clampi(sy0 - K + r, 0, H - 1);
+                    id = slab[gy * W + gx];
+                }
+                ids[sl * NRS + i] = id;
+            }
+        }
+    };
+    if (t_beg + slot_in_xcd < t_end) stage(t_beg + slot_in_xcd);
+
+    for (int tile = t_beg + slot_in_xcd; tile < t_end; tile += per_xcd) {
+        const int tx = tile % tiles_x;
+        const int ty = (tile / tiles_x) % tiles_y;
+        const int z = tile / (tiles_x * tiles_y);
+        const int x0 = tx * kTile, y0 = ty * kTile;
+
+        // 1 + 2: claim the ids staged in registers (RPI rows per instruction,
+        // one lane per column); only the first lane of a run of equal ids
+        // (DPP row shift) tries, which keeps the CAS uncontended.  The rows
+        // of the next tile are loaded while this tile is evaluated.
+#pragma unroll
+        for (int n = 0; n < 3 * NRS; ++n) {
+            const int id = ids[n];
+            const int left = __builtin_amdgcn_update_dpp(-2, id, 0x111, 0xF, 0xF, false);
+            if (id >= 0 && id != left) {
+                const unsigned sh = (unsigned)(id & 3) * 8u;
+                uint32_t old = s_slot[id >> 2];
+                while (((old >> sh) & 0xFFu) == kSlotEmpty) {
+                    const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
+                    const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
+                    if (prev == old) {
+                        const int j = atomicAdd(&s_count, 1);
+                        if (j < kListCap) s_list[j] = id;
+                        const unsigned byte = j < kMaxCand ? (unsigned)j : (unsigned)(kMaxCand - 1);
+                        atomicAnd(&s_slot[id >> 2], ~(0xFFu << sh));
+                        atomicOr(&s_slot[id >> 2], byte << sh);
+                        break;
+                    }
+                    old = prev;
+                }
+            }
+        }
+        __syncthreads();
+        const int ncand = s_count;
+        const bool overflow = ncand > kMaxCand;
+
+        // 3: compact indices of the staged ids, candidate records
+        if (!overflow) {
+#pragma unroll
+            for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+                for (int i = 0; i < NRS; ++i) {
+                    const int r = (wave + 4 * i) * RPI + sub;
+                    const int id = ids[sl * NRS + i];
+                    if (lane_ok && r < R) {
+                        unsigned ci = (unsigned)kMaxCand;
+                        if (id >= 0) ci = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
+                        s_idx[sl * RA + r * R + col] = (uint8_t)ci;
+                    }
+                }
+            if (tid < ncand) {
+                const int id = s_list[tid];
+                SeedRec rr = rec[id];
+                s_rec[tid * 3 + 0] = rr.a;
+                s_rec[tid * 3 + 1] = rr.b;
+                s_rec[tid * 3 + 2] = rr.c;
+                s_rid[tid] = id;
+            }
+        }
+        // prefetch the next tile's rows (ids is free once converted)
+        if (tile + per_xcd < t_end) stage(tile + per_xcd);
+        __syncthreads();
+
+        // 4: evaluate
+        const int lx = (wave & 1) * 8 + (lane & 7);
+        const int ly = (wave >> 1) * 8 + (lane >> 3);
+        const int x = x0 + lx, y = y0 + ly;
+        if (x < W && y < H) {
+            const size_t v = ((size_t)z * H + y) * W + x;
+            const uint32_t pk = lab[v];
+            const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+            const float fx = (float)x, fy = (float)y, fz = (float)z;
+            unsigned long long best = ASV_KEY_INIT;
+            if (!overflow) {
+                const int base = (ly + K) * R + (lx + K);
+                unsigned long long mask = 0ull;
+#pragma unroll
+                for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; ++dx)
+                            mask |= 1ull << s_idx[sl * RA + base + dy * K * R + dx * K];
+                mask &= ~(1ull << kMaxCand);
+                while (mask) {
+                    const int j = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1ull;
+                    const float d = cand_dist(s_rec[j * 3 + 0], s_rec[j * 3 + 1], s_rec[j * 3 + 2], fx, fy, fz, l0,
+                                              l1, l2, ratio);
+                    const unsigned long long kk = dkey(d, s_rid[j]);
+                    best = kk < best ? kk : best;
+                }
+            } else {
+                const int xs[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
+                const int ys[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
+                const int zs[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
+                int seen0 = -1, seen1 = -1;
+                for (int sl = 0; sl < 3; ++sl)
+                    for (int r = 0; r < 3; ++r) {
+                        const int32_t* row = in + ((size_t)zs[sl] * H + ys[r]) * W;
+                        for (int c = 0; c < 3; ++c) {
+                            const int id = row[xs[c]];
+                            if (id < 0 || id == seen0 || id == seen1) continue;
+                            seen1 = seen0;
+                            seen0 = id;
+                            const SeedRec s = rec[id];
+                            const float d = cand_dist(s.a, s.b, s.c, fx, fy, fz, l0, l1, l2, ratio);
+                            const unsigned long long kk = dkey(d, id);
+                            best = kk < best ? kk : best;
+                        }
+                    }
+            }
+            out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
+        }
+        __syncthreads();
+
+        // 5: release the claimed bytes
+        if (ncand <= kListCap) {
+            for (int j = tid; j < ncand; j += 256) {
+                const int id = s_list[j];
+                atomicOr(&s_slot[id >> 2], 0xFFu << ((unsigned)(id & 3) * 8u));
+            }
+        } else {
+            for (int i = tid; i < nwords; i += 256) s_slot[i] = 0xFFFFFFFFu;
+        }
+        if (tid == 0) s_count = 0;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // centroid reduction (centroids.vs:19-53)
 // ---------------------------------------------------------------------------
@@ -1070,6 +1277,25 @@ static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_
 {
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     int nt = tx * ty * d.D;
+    if (d.A <= kDirectIds) {
+        // persistent grid: resident blocks per CU x CUs, a multiple of 8
+        static int grid_for_k[17] = {0};
+        if (grid_for_k[K] == 0) {
+            int dev = 0, cus = 0, per_cu = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jfa_direct<K>, 256, 0);
+            int g = cus * (per_cu > 0 ? per_cu : 1);
+            g = (g / 8) * 8;
+            grid_for_k[K] = g < 8 ? 8 : g;
+        }
+        int g = grid_for_k[K];
+        int need = ((nt + 7) / 8) * 8;
+        if (g > need) g = need;
+        hipLaunchKernelGGL(k_jfa_direct<K>, dim3((unsigned)g), dim3(256), 0, st, d.W, d.H, d.D, d.A, tx, ty, nt, in,
+                           out, lab, rec, ratio);
+        return;
+    }
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
                        rec, ratio);
 }

@@ -147,6 +147,31 @@ int asv_synth_video(uint64_t seed, int32_t width, int32_t height, int32_t depth,
                     int32_t n_ellipses, float vmax, float noise_sigma,
                     uint8_t* rgb, float* fwd, float* bwd, int32_t* region);
 
+/* Host-only post-processing, Image::check_dead_seeds
+ * (GPU-AS/Image.cpp:1579-1720) with label_replace.gs and sequence_label
+ * (:1727-1778): sums as written by asv_download_sums, labels (V int32)
+ * rewritten in place; outputs the dead-seed count and final_label_num. */
+int asv_postprocess(const int64_t* sums, int32_t seeds_placed, int32_t* labels, int64_t voxels,
+                    int32_t* dead_seeds, int32_t* final_label_num);
+
+/* Image::print_sv_txt (GPU-AS/Image.cpp:1780-1806): the per-voxel label
+ * text file (top image row first, "%-4d " per label, two blank lines per
+ * frame). */
+int asv_write_label_txt(const int32_t* labels, int32_t width, int32_t height, int32_t depth,
+                        const char* path);
+
+/* Image::print_render (GPU-AS/Image.cpp:1883-1935): pseudo-colour frames as
+ * binary PPM files <prefix>NNNNN.ppm (top row first), rand() % 256 colours
+ * after srand(1), black for -1. */
+int asv_write_render_ppm(const int32_t* labels, int32_t width, int32_t height, int32_t depth,
+                         int32_t n_labels, const char* prefix);
+
+/* No counterpart in the reference (its evaluation used the external LIBSVX
+ * tools, README.md:28): 3D undersegmentation error and 3D boundary recall
+ * of labels against ground-truth regions of the same shape and row order. */
+int asv_eval_metrics(const int32_t* labels, const int32_t* regions, int32_t width, int32_t height,
+                     int32_t depth, double* undersegmentation, double* boundary_recall);
+
 const char* asv_last_error(void);
 int asv_device_count(void);
 

@@ -102,6 +102,15 @@ def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
         "frames_prepared": kw.get("frames_prepared", 0)})
 
 
+def test_many_seeds_hash_table_path(have_gpu):
+    """More seeds than the direct-mapped LDS id table holds (A > 16384):
+    the tiles use the hashed candidate table."""
+    rgb, fwd, bwd, _ = asv.synth_video(12, 640, 480, 48, 20, 3.0, 5.0)
+    got = _clip_run(rgb, fwd, bwd, 18000, 2)
+    assert got["A"] > 16384
+    _assert_run_equal(got, rgb, fwd, bwd, 18000, 2)
+
+
 def test_config1_bit_exact(have_gpu):
     """160x120x30, seed 0, 12 ellipses, K=500 (A=468), 10 iterations."""
     rgb, fwd, bwd, _ = asv.synth_video(0, 160, 120, 30, 12, 2.0, 4.0)
