@@ -51,6 +51,26 @@ SIGNATURES = {
     "asv_write_label_txt": (ctypes.c_int, [_vp, _i32, _i32, _i32, ctypes.c_char_p]),
     "asv_write_render_ppm": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, ctypes.c_char_p]),
     "asv_eval_metrics": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "asv_plan_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
+    "asv_reseed": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp]),
+    "asv_shift_frames": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "asv_upload_frames_range": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "asv_create_seeded": (ctypes.c_int, [_vp, _vp, _i32, _i32, ctypes.POINTER(_vp)]),
+    "asv_stream_default_params": (None, [_vp, _i32, _i32, _i32, _i32]),
+    "asv_stream_plan": (ctypes.c_int, [_vp, _vp, _i32]),
+    "asv_stream_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "asv_stream_destroy": (ctypes.c_int, [_vp]),
+    "asv_stream_block_count": (ctypes.c_int, [_vp]),
+    "asv_stream_seeds_placed": (ctypes.c_int, [_vp]),
+    "asv_stream_block_info": (ctypes.c_int, [_vp, _vp]),
+    "asv_stream_block": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "asv_stream_run_video": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "asv_stream_last_error": (ctypes.c_char_p, []),
+    "asv_serve_default_params": (None, [_vp]),
+    "asv_serve_footprint": (_i64, [_vp, _vp]),
+    "asv_serve_run": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp]),
+    "asv_synth_video_range": (ctypes.c_int, [ctypes.c_uint64, _i32, _i32, _i32, _i32, _f32, _f32, _i32, _i32,
+                                             _vp, _vp, _vp, _vp]),
     "asv_last_error": (ctypes.c_char_p, []),
     "asv_device_count": (ctypes.c_int, []),
 }
@@ -65,6 +85,38 @@ class Params(ctypes.Structure):
         ("tie_rule", _i32), ("angle_gate", _i32), ("schedule", _i32),
         ("frames_prepared", _i32),
     ]
+
+
+class StreamParams(ctypes.Structure):
+    """Mirror of struct asv_stream_params."""
+
+    _fields_ = [
+        ("width", _i32), ("height", _i32), ("total_frames", _i32), ("seeds_num", _i32),
+        ("seeds_per_100_frames", _i32), ("block_size", _i32), ("overlap", _i32),
+        ("lloyd_iters", _i32), ("compactness", _f32), ("angle_gate", _i32), ("tie_rule", _i32),
+        ("frames_prepared", _i32),
+    ]
+
+
+class ServeParams(ctypes.Structure):
+    _fields_ = [("memory_budget", _i64), ("max_active", _i32), ("block_size", _i32), ("overlap", _i32),
+                ("lloyd_iters", _i32), ("compactness", _f32), ("angle_gate", _i32), ("honor_arrivals", _i32)]
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("stream_id", _i32), ("width", _i32), ("height", _i32), ("frames", _i32), ("density", _f32),
+                ("n_ellipses", _i32), ("video_seed", ctypes.c_uint64), ("vmax", _f32), ("noise_sigma", _f32),
+                ("arrival_ms", ctypes.c_double), ("rgb", _vp), ("fwd", _vp), ("bwd", _vp)]
+
+
+class ServeWindow(ctypes.Structure):
+    _fields_ = [("stream_id", _i32), ("window", _i32), ("frames", _i32), ("seeds", _i32),
+                ("ready_ms", ctypes.c_double), ("start_ms", ctypes.c_double), ("end_ms", ctypes.c_double)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("stream_id", _i32), ("status", _i32), ("labels_hash", ctypes.c_uint64), ("footprint", _i64),
+                ("voxel_iters", _i64), ("admitted_ms", ctypes.c_double), ("done_ms", ctypes.c_double)]
 
 
 _lib = None

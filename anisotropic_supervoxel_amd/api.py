@@ -102,15 +102,73 @@ def eval_metrics(labels, regions):
     return ue.value, br.value
 
 
-class Clip:
-    """One clip's device context (asv_ctx)."""
+def plan_seeds(params):
+    """Grid seed voxels (A x 3, GL orientation) asv_create would place."""
+    A = plan(params)[0]
+    xyz = np.zeros((A, 3), np.int32)
+    n = _lib.load().asv_plan_seeds(ctypes.byref(params), ptr(xyz), xyz.size)
+    if n < 0:
+        check(n)
+    return xyz
 
-    def __init__(self, params, stream=None):
+
+def stream_params(width, height, total_frames, seeds_num, **kw):
+    """asv_stream_default_params, then keyword overrides."""
+    p = _lib.StreamParams()
+    _lib.load().asv_stream_default_params(ctypes.byref(p), width, height, total_frames, seeds_num)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown stream parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def _check_stream(rc):
+    if rc != _lib.ASV_OK:
+        lib = _lib.load()
+        raise AsvError(f"{_lib.ERRORS.get(rc, rc)}: {lib.asv_stream_last_error().decode()} "
+                       f"{lib.asv_last_error().decode()}")
+
+
+def stream_plan(sp):
+    """Block intervals [(start, end)] of the temporal-block run."""
+    iv = np.zeros(2 * 4096, np.int32)
+    n = _lib.load().asv_stream_plan(ctypes.byref(sp), ptr(iv), 4096)
+    if n < 0:
+        _check_stream(n)
+    return [(int(iv[2 * i]), int(iv[2 * i + 1])) for i in range(n)]
+
+
+def stream_run_video(sp, rgb, fwd, bwd, stream=None):
+    """Whole video through the temporal blocks; returns (labels [D,H,W] global
+    ids in GL rows, seeds placed)."""
+    D, H, W = sp.total_frames, sp.height, sp.width
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    fwd = np.ascontiguousarray(fwd, np.float32)
+    bwd = np.ascontiguousarray(bwd, np.float32)
+    assert rgb.shape == (D, H, W, 3) and fwd.shape == (D, H, W, 2) and bwd.shape == (D, H, W, 2)
+    labels = np.full((D, H, W), -7, np.int32)
+    placed = ctypes.c_int32(0)
+    _check_stream(_lib.load().asv_stream_run_video(ctypes.byref(sp), ptr(rgb), ptr(fwd), ptr(bwd), ptr(labels),
+                                                   ctypes.byref(placed), ctypes.c_void_p(stream) if stream else None))
+    return labels, placed.value
+
+
+class Clip:
+    """One clip's device context (asv_ctx).  With seeds_xyz (n x 3) and
+    grid_step, the initial seeds are given instead of the grid rule."""
+
+    def __init__(self, params, stream=None, seeds_xyz=None, grid_step=None):
         self.lib = _lib.load()
         self.p = params
         self.stream = ctypes.c_void_p(stream) if stream else None
         h = ctypes.c_void_p()
-        check(self.lib.asv_create(ctypes.byref(params), ctypes.byref(h)))
+        if seeds_xyz is None:
+            check(self.lib.asv_create(ctypes.byref(params), ctypes.byref(h)))
+        else:
+            xyz = np.ascontiguousarray(seeds_xyz, np.int32)
+            check(self.lib.asv_create_seeded(ctypes.byref(params), ptr(xyz), len(xyz), int(grid_step),
+                                             ctypes.byref(h)))
         self.h = h
         self.A = self.lib.asv_seeds_placed(h)
         self.s = self.lib.asv_grid_step(h)

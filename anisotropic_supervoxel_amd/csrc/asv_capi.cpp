@@ -147,6 +147,7 @@ struct asv_ctx {
     std::vector<int> steps;
     std::vector<float> mult;
     std::vector<int> grid_host;
+    std::vector<int4> grid_staging;
     int64_t bytes = 0;
     bool have_frames = false, have_flow = false, prepared = false;
     bool trace_on = false;
@@ -169,6 +170,8 @@ struct asv_ctx {
     unsigned long long* sums[2] = {nullptr, nullptr};
     float* trace = nullptr;
 
+    int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
+    int depth_cap = 0;      // frames the volume buffers hold
     int64_t V() const { return (int64_t)d.W * d.H * d.D; }
 };
 
@@ -265,19 +268,24 @@ int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step, int
     return (int)st.size();
 }
 
-int asv_create(const asv_params* p, asv_ctx** out)
+}  // extern "C"
+
+namespace {
+
+// Allocates a context for p whose initial seeds are xyz (A x 3) with grid
+// step s (ratio and schedule follow from s).
+int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out, int seed_cap = 0)
 {
-    if (!out) return fail(ASV_ERR_ARG, "null output pointer");
-    *out = nullptr;
-    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    const int A = (int)(xyz.size() / 3);
+    const int Acap = seed_cap > A ? seed_cap : A;
     asv_ctx* c = new asv_ctx();
     c->p = *p;
-    int s = 0;
-    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &c->grid_host, &s);
-    if (A <= 0) { delete c; return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes"); }
+    c->grid_host = std::move(xyz);
     if (s < 1) s = 1;
     c->s = s;
     c->d = Dims{p->width, p->height, p->depth, A};
+    c->seed_cap = Acap;
+    c->depth_cap = p->depth;
     c->ratio = (p->compactness / (float)s) * (p->compactness / (float)s);   // Image.cpp:1051
     schedule_list(p->schedule, s, p->width, c->steps, c->mult);
 
@@ -294,12 +302,12 @@ int asv_create(const asv_params* p, asv_ctx** out)
     if (e == hipSuccess) e = dalloc(&c->fix_count, 4, &tb);
     if (e == hipSuccess) e = dalloc(&c->labels[0], (size_t)V, &tb);
     if (e == hipSuccess) e = dalloc(&c->labels[1], (size_t)V, &tb);
-    if (e == hipSuccess) e = dalloc(&c->grid, (size_t)A, &tb);
-    if (e == hipSuccess) e = dalloc(&c->pos, (size_t)A, &tb);
-    if (e == hipSuccess) e = dalloc(&c->slab, (size_t)A, &tb);
-    if (e == hipSuccess) e = dalloc(&c->rec, (size_t)A, &tb);
-    if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(A + 1) * kSumWords, &tb);
-    if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(A + 1) * kSumWords, &tb);
+    if (e == hipSuccess) e = dalloc(&c->grid, (size_t)Acap, &tb);
+    if (e == hipSuccess) e = dalloc(&c->pos, (size_t)Acap, &tb);
+    if (e == hipSuccess) e = dalloc(&c->slab, (size_t)Acap, &tb);
+    if (e == hipSuccess) e = dalloc(&c->rec, (size_t)Acap, &tb);
+    if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(Acap + 1) * kSumWords, &tb);
+    if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(Acap + 1) * kSumWords, &tb);
     if (e != hipSuccess) {
         free_ctx(c);
         delete c;
@@ -319,6 +327,38 @@ int asv_create(const asv_params* p, asv_ctx** out)
     }
     *out = c;
     return ASV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asv_create(const asv_params* p, asv_ctx** out)
+{
+    if (!out) return fail(ASV_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    int s = 0;
+    std::vector<int> xyz;
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &xyz, &s);
+    if (A <= 0) return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes");
+    return create_ctx(p, std::move(xyz), s, out);
+}
+
+int asv_create_seeded(const asv_params* p, const int32_t* xyz, int32_t n, int32_t grid_step, asv_ctx** out)
+{
+    if (!out) return fail(ASV_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    if (!xyz || n < 1 || grid_step < 1) return fail(ASV_ERR_ARG, "need at least one seed and a grid step >= 1");
+    std::vector<int> v((size_t)n * 3);
+    for (int i = 0; i < n; ++i) {
+        int x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (x < 0 || x >= p->width || y < 0 || y >= p->height || z < 0 || z >= p->depth)
+            return fail(ASV_ERR_ARG, "seed voxel outside the clip");
+        v[3 * i] = x; v[3 * i + 1] = y; v[3 * i + 2] = z;
+    }
+    return create_ctx(p, std::move(v), grid_step, out, p->seeds_num);
 }
 
 int asv_destroy(asv_ctx* c)
@@ -344,6 +384,16 @@ int asv_schedule(const asv_ctx* c, int32_t* steps, float* mult, int32_t cap)
     return n;
 }
 
+int asv_plan_seeds(const asv_params* p, int32_t* xyz, int32_t cap)
+{
+    if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
+    std::vector<int> v;
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &v, nullptr);
+    if (A <= 0) return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes");
+    for (int i = 0; i < A * 3 && i < cap; ++i) xyz[i] = v[(size_t)i];
+    return A;
+}
+
 int asv_grid_seeds(const asv_ctx* c, int32_t* xyz, int32_t cap)
 {
     if (!c) return fail(ASV_ERR_ARG, "null context");
@@ -357,6 +407,61 @@ int asv_upload_frames(asv_ctx* c, const uint8_t* rgb, void* stream)
     if (!c || !rgb) return fail(ASV_ERR_ARG, "null argument");
     HIP_TRY(hipMemcpyAsync(c->rgb, rgb, (size_t)c->V() * 3, hipMemcpyDefault, S(stream)));
     c->have_frames = true;
+    c->prepared = false;
+    return ASV_OK;
+}
+
+int asv_reseed(asv_ctx* c, const int32_t* xyz, int32_t n, int32_t grid_step, int32_t depth, void* stream)
+{
+    if (!c || !xyz || n < 1 || grid_step < 1) return fail(ASV_ERR_ARG, "need a context, seeds and a grid step");
+    if (n > c->seed_cap) return fail(ASV_ERR_ARG, "more seeds than the context holds");
+    if (depth < 1 || depth > c->depth_cap) return fail(ASV_ERR_ARG, "depth outside the context's capacity");
+    for (int i = 0; i < n; ++i)
+        if (xyz[3 * i] < 0 || xyz[3 * i] >= c->d.W || xyz[3 * i + 1] < 0 || xyz[3 * i + 1] >= c->d.H ||
+            xyz[3 * i + 2] < 0 || xyz[3 * i + 2] >= depth)
+            return fail(ASV_ERR_ARG, "seed voxel outside the clip");
+    c->p.depth = depth;
+    c->d.D = depth;
+    c->d.A = n;
+    c->s = grid_step;
+    c->ratio = (c->p.compactness / (float)grid_step) * (c->p.compactness / (float)grid_step);
+    schedule_list(c->p.schedule, grid_step, c->d.W, c->steps, c->mult);
+    c->grid_host.assign(xyz, xyz + 3 * (size_t)n);
+    c->grid_staging.resize((size_t)n);
+    for (int i = 0; i < n; ++i) c->grid_staging[(size_t)i] = make_int4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0);
+    HIP_TRY(hipMemcpyAsync(c->grid, c->grid_staging.data(), sizeof(int4) * (size_t)n, hipMemcpyHostToDevice, S(stream)));
+    c->prepared = false;
+    c->last_iter = 0;
+    if (c->trace) { (void)hipFree(c->trace); c->trace = nullptr; c->trace_on = false; }
+    return ASV_OK;
+}
+
+int asv_shift_frames(asv_ctx* c, int32_t src_frame, int32_t n_frames, void* stream)
+{
+    if (!c || src_frame < 0 || n_frames < 0 || src_frame + n_frames > c->depth_cap || n_frames > src_frame)
+        return fail(ASV_ERR_ARG, "frame ranges must lie in the context and not overlap");
+    if (n_frames == 0) return ASV_OK;
+    const size_t plane = (size_t)c->d.W * c->d.H;
+    const size_t f0 = (size_t)src_frame * plane, n = (size_t)n_frames * plane;
+    HIP_TRY(hipMemcpyAsync(c->rgb, c->rgb + f0 * 3, n * 3, hipMemcpyDeviceToDevice, S(stream)));
+    HIP_TRY(hipMemcpyAsync(c->fwd, c->fwd + f0, n * 8, hipMemcpyDeviceToDevice, S(stream)));
+    HIP_TRY(hipMemcpyAsync(c->bwd, c->bwd + f0, n * 8, hipMemcpyDeviceToDevice, S(stream)));
+    c->prepared = false;
+    return ASV_OK;
+}
+
+int asv_upload_frames_range(asv_ctx* c, int32_t first, int32_t count, const uint8_t* rgb, const float* fwd,
+                            const float* bwd, void* stream)
+{
+    if (!c || !rgb || !fwd || !bwd || first < 0 || count < 0 || first + count > c->depth_cap)
+        return fail(ASV_ERR_ARG, "frame range outside the context");
+    const size_t plane = (size_t)c->d.W * c->d.H;
+    const size_t f0 = (size_t)first * plane, n = (size_t)count * plane;
+    HIP_TRY(hipMemcpyAsync(c->rgb + f0 * 3, rgb, n * 3, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipMemcpyAsync(c->fwd + f0, fwd, n * 8, hipMemcpyDefault, S(stream)));
+    HIP_TRY(hipMemcpyAsync(c->bwd + f0, bwd, n * 8, hipMemcpyDefault, S(stream)));
+    c->have_frames = true;
+    c->have_flow = true;
     c->prepared = false;
     return ASV_OK;
 }

@@ -1,0 +1,267 @@
+// asv_serve.cpp -- the served streaming form: many concurrent video
+// streams, each processed as sliding windows (a temporal-block run with
+// seeds carried across windows, as the streamGPU-AS program does per block,
+// streamGPU-AS/Render.cpp:249-277), with continuous batching on one GPU.
+//
+// A dispatcher admits requests in arrival order as a stream slot and device
+// memory (the stream's resident window footprint) free up; a request that
+// does not fit waits while later, smaller ones may go first.  Each admitted
+// stream runs on a worker thread with its own HIP stream, so windows of
+// different streams overlap on the device.  Per window the loop records
+// when it became ready (frames present and the previous window done), when
+// its GPU work was enqueued and when its labels were back.
+#include "../../include/asv.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using clock_t_ = std::chrono::steady_clock;
+
+double ms_since(clock_t_::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(clock_t_::now() - t0).count();
+}
+
+uint64_t fnv1a(const int32_t* p, size_t n, uint64_t h)
+{
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n * 4; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+asv_stream_params stream_params_of(const asv_serve_params* sp, const asv_serve_request* r)
+{
+    asv_stream_params p;
+    asv_stream_default_params(&p, r->width, r->height, r->frames, 1);
+    // density d supervoxels per 1000 pixels per frame = d*W*H/10 per 100 frames
+    p.seeds_per_100_frames = (int32_t)((double)r->density * r->width * r->height / 10.0);
+    if (p.seeds_per_100_frames < 1) p.seeds_per_100_frames = 1;
+    p.block_size = sp->block_size;
+    p.overlap = sp->overlap;
+    p.lloyd_iters = sp->lloyd_iters;
+    p.compactness = sp->compactness;
+    p.angle_gate = sp->angle_gate;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void asv_serve_default_params(asv_serve_params* p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->memory_budget = (int64_t)64 << 30;
+    p->max_active = 8;
+    p->block_size = 10;
+    p->overlap = 5;
+    p->lloyd_iters = 5;
+    p->compactness = 6.0f;
+    p->angle_gate = 0;
+    p->honor_arrivals = 1;
+}
+
+int64_t asv_serve_footprint(const asv_serve_params* sp, const asv_serve_request* r)
+{
+    if (!sp || !r || r->width < 2 || r->height < 2 || r->frames < 1) return -1;
+    asv_stream_params p = stream_params_of(sp, r);
+    int32_t iv[2 * 1024];
+    int nb = asv_stream_plan(&p, iv, 1024);
+    if (nb < 1) return -1;
+    int depth = 0;
+    for (int b = 0; b < nb && b < 1024; ++b) depth = std::max(depth, iv[2 * b + 1] - iv[2 * b]);
+    const int64_t V = (int64_t)r->width * r->height * depth;
+    const int64_t seeds = 4 * ((int64_t)p.seeds_per_100_frames * p.block_size / 100) + 64;
+    // frames, flows, LAB, metric field, fixup list, two label volumes; seeds
+    return V * (3 + 16 + 4 + 32 + 4 + 8) + seeds * 200 + (1 << 20);
+}
+
+int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int32_t n, asv_serve_result* results,
+                  asv_serve_window* windows, int32_t window_cap, int32_t* n_windows, double* wall_ms)
+{
+    if (!sp || (!reqs && n > 0) || !results || sp->max_active < 1 || sp->memory_budget <= 0) return ASV_ERR_ARG;
+    int dev = 0;
+    (void)hipGetDevice(&dev);     // without a device every admitted stream fails with a HIP error
+
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return reqs[a].arrival_ms < reqs[b].arrival_ms; });
+    for (int i = 0; i < n; ++i) {
+        std::memset(&results[i], 0, sizeof(results[i]));
+        results[i].stream_id = reqs[i].stream_id;
+        results[i].status = ASV_ERR_STATE;
+    }
+
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int> work;            // admitted request indices
+    int active = 0;
+    int64_t active_bytes = 0;
+    bool closing = false;
+    int nwin = 0;
+    const auto t0 = clock_t_::now();
+
+    auto record = [&](const asv_serve_window& w) {
+        std::lock_guard<std::mutex> g(mu);
+        if (windows && nwin < window_cap) windows[nwin] = w;
+        ++nwin;
+    };
+
+    auto process = [&](int ri, hipStream_t st) -> int {
+        const asv_serve_request& r = reqs[ri];
+        asv_stream_params p = stream_params_of(sp, &r);
+        asv_stream* s = nullptr;
+        int rc = asv_stream_create(&p, &s);
+        if (rc) return rc;
+        const int nb = asv_stream_block_count(s);
+        std::vector<int32_t> iv((size_t)nb * 2);
+        asv_stream_plan(&p, iv.data(), nb);
+        const size_t plane = (size_t)r.width * r.height;
+        std::vector<int32_t> labels(plane * (size_t)r.frames, -1);
+        uint64_t vox_iters = 0;
+        double ready = ms_since(t0);
+        for (int b = 0; b < nb && !rc; ++b) {
+            const int f0 = iv[2 * (size_t)b], f1 = iv[2 * (size_t)b + 1];
+            const size_t nf = (size_t)(f1 - f0);
+            std::vector<uint8_t> rgb_buf;
+            std::vector<float> fwd_buf, bwd_buf;
+            const uint8_t* rgb;
+            const float *fwd, *bwd;
+            if (r.rgb && r.fwd && r.bwd) {
+                rgb = r.rgb + (size_t)f0 * plane * 3;
+                fwd = r.fwd + (size_t)f0 * plane * 2;
+                bwd = r.bwd + (size_t)f0 * plane * 2;
+            } else {
+                rgb_buf.resize(plane * nf * 3);
+                fwd_buf.resize(plane * nf * 2);
+                bwd_buf.resize(plane * nf * 2);
+                rc = asv_synth_video_range(r.video_seed, r.width, r.height, r.frames, r.n_ellipses, r.vmax,
+                                           r.noise_sigma, f0, f1, rgb_buf.data(), fwd_buf.data(), bwd_buf.data(),
+                                           nullptr);
+                if (rc) break;
+                rgb = rgb_buf.data();
+                fwd = fwd_buf.data();
+                bwd = bwd_buf.data();
+            }
+            // the window is ready once its frames are present and the previous window is done
+            ready = std::max(ready, ms_since(t0));
+            asv_serve_window w;
+            w.stream_id = r.stream_id;
+            w.window = b;
+            w.frames = (int32_t)nf;
+            w.ready_ms = ready;
+            w.start_ms = ms_since(t0);
+            int32_t done = 0;
+            rc = asv_stream_block(s, b, rgb, fwd, bwd, labels.data(), &done, st);
+            w.end_ms = ms_since(t0);
+            int32_t info[6];
+            asv_stream_block_info(s, info);
+            w.seeds = info[1];
+            record(w);
+            ready = w.end_ms;
+            vox_iters += (uint64_t)plane * nf * (uint64_t)p.lloyd_iters;
+        }
+        if (!rc) {
+            results[ri].labels_hash = fnv1a(labels.data(), labels.size(), 1469598103934665603ull);
+            results[ri].voxel_iters = (int64_t)vox_iters;
+        }
+        asv_stream_destroy(s);
+        return rc;
+    };
+
+    auto worker = [&]() {
+        (void)hipSetDevice(dev);
+        hipStream_t st = nullptr;
+        (void)hipStreamCreate(&st);
+        for (;;) {
+            int ri;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return closing || !work.empty(); });
+                if (work.empty()) break;
+                ri = work.front();
+                work.pop_front();
+            }
+            int rc = process(ri, st);
+            {
+                std::lock_guard<std::mutex> g(mu);
+                results[ri].status = rc;
+                results[ri].done_ms = ms_since(t0);
+                --active;
+                active_bytes -= results[ri].footprint;
+            }
+            cv.notify_all();
+        }
+        if (st) (void)hipStreamDestroy(st);
+    };
+
+    std::vector<std::thread> pool;
+    for (int i = 0; i < sp->max_active; ++i) pool.emplace_back(worker);
+
+    std::deque<int> pending(order.begin(), order.end());
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        while (!pending.empty()) {
+            const double now = ms_since(t0);
+            double next_arrival = -1.0;
+            for (auto it = pending.begin(); it != pending.end();) {
+                const int ri = *it;
+                const asv_serve_request& r = reqs[ri];
+                if (sp->honor_arrivals && r.arrival_ms > now) {
+                    if (next_arrival < 0.0 || r.arrival_ms < next_arrival) next_arrival = r.arrival_ms;
+                    ++it;
+                    continue;
+                }
+                const int64_t fp = asv_serve_footprint(sp, &r);
+                if (fp < 0 || fp > sp->memory_budget) {          // can never be admitted
+                    results[ri].status = ASV_ERR_ADMIT;
+                    results[ri].footprint = fp;
+                    it = pending.erase(it);
+                    continue;
+                }
+                if (active < sp->max_active && active_bytes + fp <= sp->memory_budget) {
+                    ++active;
+                    active_bytes += fp;
+                    results[ri].footprint = fp;
+                    results[ri].admitted_ms = now;
+                    work.push_back(ri);
+                    it = pending.erase(it);
+                    cv.notify_all();
+                    continue;
+                }
+                ++it;
+            }
+            if (pending.empty()) break;
+            if (next_arrival >= 0.0) {
+                const double wait = std::max(0.0, next_arrival - ms_since(t0));
+                cv.wait_for(lk, std::chrono::microseconds((int64_t)(wait * 1000.0) + 1));
+            } else {
+                cv.wait(lk);
+            }
+        }
+        closing = true;
+    }
+    cv.notify_all();
+    for (auto& th : pool) th.join();
+    if (n_windows) *n_windows = nwin;
+    if (wall_ms) *wall_ms = ms_since(t0);
+    for (int i = 0; i < n; ++i)
+        if (results[i].status != ASV_OK) return results[i].status == ASV_ERR_ADMIT ? ASV_OK : results[i].status;
+    return ASV_OK;
+}
+
+}  // extern "C"

@@ -57,10 +57,11 @@ struct Ellipse {
 
 }  // namespace
 
-extern "C" int asv_synth_video(uint64_t seed, int32_t W, int32_t H, int32_t D, int32_t n_ell, float vmax,
-                               float sigma, uint8_t* rgb, float* fwd, float* bwd, int32_t* region)
+extern "C" int asv_synth_video_range(uint64_t seed, int32_t W, int32_t H, int32_t D, int32_t n_ell, float vmax,
+                                     float sigma, int32_t t0, int32_t t1, uint8_t* rgb, float* fwd, float* bwd,
+                                     int32_t* region)
 {
-    if (W < 2 || H < 2 || D < 1 || n_ell < 0) return ASV_ERR_ARG;
+    if (W < 2 || H < 2 || D < 1 || n_ell < 0 || t0 < 0 || t1 > D || t0 >= t1) return ASV_ERR_ARG;
     const double kPi = 3.14159265358979323846;
     SplitMix64 g(seed);
     double corner[4][3];
@@ -85,15 +86,15 @@ extern "C" int asv_synth_video(uint64_t seed, int32_t W, int32_t H, int32_t D, i
         E.vy = (double)vmax * (2.0 * g.uniform() - 1.0);
     }
 
-    std::atomic<int> next_frame(0);
+    std::atomic<int> next_frame(t0);
     auto worker = [&]() {
-    for (int t = next_frame++; t < D; t = next_frame++) {
+    for (int t = next_frame++; t < t1; t = next_frame++) {
         SplitMix64 nz(seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(t + 1)));
         double spare = 0.0;
         bool have_spare = false;
         for (int py = 0; py < H; ++py) {
             for (int px = 0; px < W; ++px) {
-                size_t v = ((size_t)t * H + py) * W + px;
+                size_t v = ((size_t)(t - t0) * H + py) * W + px;
                 double cxp = px + 0.5, cyp = py + 0.5;
                 int top = -1;
                 for (int e = n_ell - 1; e >= 0; --e) {
@@ -138,10 +139,16 @@ extern "C" int asv_synth_video(uint64_t seed, int32_t W, int32_t H, int32_t D, i
     };
     unsigned nt = std::thread::hardware_concurrency();
     nt = nt == 0 ? 1 : (nt > 16 ? 16 : nt);
-    if ((int)nt > D) nt = (unsigned)D;
+    if ((int)nt > t1 - t0) nt = (unsigned)(t1 - t0);
     std::vector<std::thread> pool;
     for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
     worker();
     for (auto& th : pool) th.join();
     return ASV_OK;
+}
+
+extern "C" int asv_synth_video(uint64_t seed, int32_t W, int32_t H, int32_t D, int32_t n_ell, float vmax,
+                               float sigma, uint8_t* rgb, float* fwd, float* bwd, int32_t* region)
+{
+    return asv_synth_video_range(seed, W, H, D, n_ell, vmax, sigma, 0, D, rgb, fwd, bwd, region);
 }

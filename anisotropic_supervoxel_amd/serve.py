@@ -1,0 +1,118 @@
+"""Served streaming form: many concurrent video streams as sliding windows
+with seed carry, continuous batching on one GPU (asv_serve_run).
+
+Workload of the served benchmark: Poisson arrivals at 40 streams/s; each
+stream is a synthetic video (seed = stream id) with resolution uniform from
+{320x240, 640x480, 1280x720}, 60-300 frames and supervoxel density uniform
+in [1, 4] per 1000 pixels per frame; windows of 10 frames, stride 5,
+5 Lloyd iterations per window.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ServeParams, ServeRequest, ServeResult, ServeWindow
+
+RESOLUTIONS = [(320, 240), (640, 480), (1280, 720)]
+
+
+def default_params(**kw):
+    p = ServeParams()
+    _lib.load().asv_serve_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(f"unknown serve parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+def make_requests(n, rate_per_s=40.0, seed=0, resolutions=None, frames=(60, 300), density=(1.0, 4.0),
+                  ellipses=(8, 32)):
+    """n stream requests with Poisson arrival times (ms)."""
+    rng = np.random.default_rng(seed)
+    res = resolutions or RESOLUTIONS
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1000.0 / rate_per_s)
+        W, H = res[int(rng.integers(0, len(res)))]
+        out.append(dict(stream_id=i, width=W, height=H, frames=int(rng.integers(frames[0], frames[1] + 1)),
+                        density=float(rng.uniform(*density)), n_ellipses=int(rng.integers(ellipses[0], ellipses[1] + 1)),
+                        video_seed=i, vmax=2.0, noise_sigma=4.0, arrival_ms=t))
+    return out
+
+
+_FIELDS = [f[0] for f in ServeRequest._fields_ if f[0] not in ("rgb", "fwd", "bwd")]
+
+
+def _request(r):
+    q = ServeRequest(**{k: r[k] for k in _FIELDS})
+    frames = r.get("frames_data")
+    if frames is not None:
+        rgb, fwd, bwd = frames
+        q.rgb, q.fwd, q.bwd = rgb.ctypes.data, fwd.ctypes.data, bwd.ctypes.data
+    return q
+
+
+def attach_frames(requests):
+    """Generates every stream's frames and flows up front (host memory), so
+    the serving loop reads them as arrived input instead of synthesising."""
+    from .api import synth_video
+    for r in requests:
+        rgb, fwd, bwd, _ = synth_video(r["video_seed"], r["width"], r["height"], r["frames"], r["n_ellipses"],
+                                       r["vmax"], r["noise_sigma"])
+        r["frames_data"] = (rgb, fwd, bwd)
+    return requests
+
+
+def footprint(params, req):
+    r = _request(req)
+    return _lib.load().asv_serve_footprint(ctypes.byref(params), ctypes.byref(r))
+
+
+def run(params, requests):
+    """Runs the serving loop.  Returns (results list of dicts, windows as a
+    structured numpy array, wall ms)."""
+    lib = _lib.load()
+    n = len(requests)
+    reqs = (ServeRequest * max(n, 1))()
+    for i, r in enumerate(requests):
+        reqs[i] = _request(r)
+    res = (ServeResult * max(n, 1))()
+    cap = sum(max(1, r["frames"]) for r in requests) + 16
+    wins = (ServeWindow * cap)()
+    nw = ctypes.c_int32(0)
+    wall = ctypes.c_double(0.0)
+    rc = lib.asv_serve_run(ctypes.byref(params), reqs, n, res, wins, cap, ctypes.byref(nw), ctypes.byref(wall))
+    if rc != 0:
+        raise _lib.AsvError(f"asv_serve_run failed ({rc}): {lib.asv_last_error().decode()} "
+                            f"{lib.asv_stream_last_error().decode()}")
+    results = [dict(stream_id=res[i].stream_id, status=res[i].status, labels_hash=res[i].labels_hash,
+                    footprint=res[i].footprint, voxel_iters=res[i].voxel_iters, admitted_ms=res[i].admitted_ms,
+                    done_ms=res[i].done_ms) for i in range(n)]
+    dt = np.dtype([("stream_id", "i4"), ("window", "i4"), ("frames", "i4"), ("seeds", "i4"),
+                   ("ready_ms", "f8"), ("start_ms", "f8"), ("end_ms", "f8")])
+    w = np.frombuffer(bytes(wins), dtype=dt, count=cap)[: min(nw.value, cap)].copy()
+    return results, w, wall.value
+
+
+def summarize(results, windows, wall_ms):
+    lat = windows["end_ms"] - windows["ready_ms"]
+    vox = sum(r["voxel_iters"] for r in results if r["status"] == 0)
+    return {
+        "streams": len(results), "windows": int(len(windows)),
+        "p50_window_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_window_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+        "wall_ms": wall_ms, "voxel_iters": int(vox),
+        "mvoxel_iters_per_s": vox / (wall_ms * 1e-3) / 1e6 if wall_ms > 0 else None,
+    }
+
+
+def fnv1a_labels(labels):
+    """FNV-1a 64 over int32 labels (the hash asv_serve_run reports)."""
+    h = 1469598103934665603
+    for b in np.ascontiguousarray(labels, np.int32).tobytes():
+        h ^= b
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h

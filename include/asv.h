@@ -77,6 +77,31 @@ int asv_schedule(const asv_ctx* ctx, int32_t* steps, float* mult, int32_t cap);
 int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step,
              int32_t* steps, float* mult, int32_t cap);
 
+/* Host-only: the grid seed voxels asv_create would place for p (A x 3
+ * int32), GPU-AS/Image.cpp:20-92; returns A. */
+int asv_plan_seeds(const asv_params* p, int32_t* xyz, int32_t cap);
+
+/* A context whose initial seeds are given (n x 3 voxel coordinates, GL
+ * orientation, ids 0..n-1 in that order) with grid step grid_step setting
+ * ratio and the JFA schedule: the streaming blocks' kept ++ new seeds
+ * (streamGPU-AS/Render.cpp:434-441, :470-504). */
+int asv_create_seeded(const asv_params* p, const int32_t* xyz, int32_t n, int32_t grid_step, asv_ctx** out);
+
+/* Reuse a context for a new window without reallocating: new initial seeds
+ * (n <= the seed capacity, which is max(n, p->seeds_num) at creation by
+ * asv_create_seeded), grid step and depth (<= the depth it was created
+ * with).  No counterpart in the reference, which rebuilds its GL objects per
+ * block (streamGPU-AS/Render.cpp:429, :566); serves the resident windows of
+ * the streaming and served forms. */
+int asv_reseed(asv_ctx* ctx, const int32_t* xyz, int32_t n, int32_t grid_step, int32_t depth, void* stream);
+/* Device-side move of frames [src_frame, src_frame + n_frames) to [0, n_frames)
+ * of the frame and flow buffers (the overlap kept between blocks,
+ * streamGPU-AS/Render.cpp:545-556); ranges must not overlap. */
+int asv_shift_frames(asv_ctx* ctx, int32_t src_frame, int32_t n_frames, void* stream);
+/* Upload frames [first, first + count) of frames and both flows. */
+int asv_upload_frames_range(asv_ctx* ctx, int32_t first, int32_t count, const uint8_t* rgb, const float* fwd,
+                            const float* bwd, void* stream);
+
 /* Grid seed voxels (A x 3 int32, x y z in GL orientation). */
 int asv_grid_seeds(const asv_ctx* ctx, int32_t* xyz, int32_t cap);
 
@@ -171,6 +196,102 @@ int asv_write_render_ppm(const int32_t* labels, int32_t width, int32_t height, i
  * of labels against ground-truth regions of the same shape and row order. */
 int asv_eval_metrics(const int32_t* labels, const int32_t* regions, int32_t width, int32_t height,
                      int32_t depth, double* undersegmentation, double* boundary_recall);
+
+/* ---- Temporal blocks (the streamGPU-AS program) ---------------------- */
+
+typedef struct asv_stream_params {
+    int32_t width, height;
+    int32_t total_frames;           /* frames of the whole video (depth_all) */
+    int32_t seeds_num;              /* K for the whole video */
+    int32_t seeds_per_100_frames;   /* 0: K*100/total_frames (streamGPU-AS/Render.cpp:156-158) */
+    int32_t block_size;             /* 30 (streamGPU-AS/Render.cpp:199) */
+    int32_t overlap;                /* 10, frames shared with the next block (:200) */
+    int32_t lloyd_iters;            /* 20 (:252) */
+    float   compactness;            /* 6 (streamGPU-AS/Image.cpp:968) */
+    int32_t angle_gate;             /* 0: the streaming program has no gate */
+    int32_t tie_rule;
+    int32_t frames_prepared;
+} asv_stream_params;
+
+typedef struct asv_stream asv_stream;
+
+void asv_stream_default_params(asv_stream_params* p, int32_t width, int32_t height, int32_t total_frames,
+                               int32_t seeds_num);
+/* Host-only: block intervals [start, end) (streamGPU-AS/Render.cpp:198-241);
+ * returns the block count. */
+int asv_stream_plan(const asv_stream_params* p, int32_t* intervals, int32_t cap);
+int asv_stream_create(const asv_stream_params* p, asv_stream** out);
+int asv_stream_destroy(asv_stream* s);
+int asv_stream_block_count(const asv_stream* s);
+int asv_stream_seeds_placed(const asv_stream* s);
+/* After a block: {next block, A, kept seeds carried out, dead seeds,
+ * new-seed budget, seeds placed so far}. */
+int asv_stream_block_info(const asv_stream* s, int32_t* info6);
+/* Block `block` (in order): prepare_data_seeds + block_segment +
+ * post_and_next (streamGPU-AS/Render.cpp:348-569).  rgb/fwd/bwd hold the
+ * block's frames only (image rows); its finished frames are written to
+ * labels_all (W x H x total_frames, GL rows, global ids). */
+int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
+                     int32_t* labels_all, int32_t* frames_done, void* stream);
+/* All blocks of a whole video (arrays cover total_frames frames). */
+int asv_stream_run_video(const asv_stream_params* p, const uint8_t* rgb, const float* fwd, const float* bwd,
+                         int32_t* labels_all, int32_t* seeds_placed, void* stream);
+const char* asv_stream_last_error(void);
+
+/* ---- Served streaming ------------------------------------------------- */
+/* No single counterpart in the reference (its streaming program handles one
+ * video, streamGPU-AS/Render.cpp:100-300); the served loop runs many streams'
+ * temporal blocks concurrently on one GPU with admission by footprint. */
+
+typedef struct asv_serve_params {
+    int64_t memory_budget;      /* device bytes the admitted streams' windows may hold */
+    int32_t max_active;         /* streams in flight (worker threads, one HIP stream each) */
+    int32_t block_size;         /* window frames (10) */
+    int32_t overlap;            /* frames shared by consecutive windows (5: stride 5) */
+    int32_t lloyd_iters;        /* per window (5) */
+    float   compactness;        /* 6 */
+    int32_t angle_gate;         /* 0 */
+    int32_t honor_arrivals;     /* 1: a request is eligible from arrival_ms of wall time */
+} asv_serve_params;
+
+typedef struct asv_serve_request {
+    int32_t stream_id;
+    int32_t width, height, frames;
+    float   density;            /* supervoxels per 1000 pixels per frame */
+    int32_t n_ellipses;
+    uint64_t video_seed;        /* synthetic source of the stream's frames */
+    float   vmax, noise_sigma;
+    double  arrival_ms;
+    /* the stream's frames and flows in host memory (frames x H x W x 3 / x 2,
+     * image rows); when null they are generated from video_seed per window */
+    const uint8_t* rgb;
+    const float* fwd;
+    const float* bwd;
+} asv_serve_request;
+
+typedef struct asv_serve_window {
+    int32_t stream_id, window, frames, seeds;
+    double  ready_ms, start_ms, end_ms;     /* since the run started */
+} asv_serve_window;
+
+typedef struct asv_serve_result {
+    int32_t stream_id;
+    int32_t status;             /* ASV_OK, ASV_ERR_ADMIT (never fits the budget) or an error */
+    uint64_t labels_hash;       /* FNV-1a 64 of the stitched global labels (GL rows) */
+    int64_t footprint;          /* device bytes admitted for the stream */
+    int64_t voxel_iters;
+    double  admitted_ms, done_ms;
+} asv_serve_result;
+
+void asv_serve_default_params(asv_serve_params* p);
+int64_t asv_serve_footprint(const asv_serve_params* p, const asv_serve_request* r);
+int asv_serve_run(const asv_serve_params* p, const asv_serve_request* reqs, int32_t n, asv_serve_result* results,
+                  asv_serve_window* windows, int32_t window_cap, int32_t* n_windows, double* wall_ms);
+
+/* Frames [t0, t1) of the synthetic video (same content as asv_synth_video). */
+int asv_synth_video_range(uint64_t seed, int32_t width, int32_t height, int32_t depth, int32_t n_ellipses,
+                          float vmax, float noise_sigma, int32_t t0, int32_t t1, uint8_t* rgb, float* fwd,
+                          float* bwd, int32_t* region);
 
 const char* asv_last_error(void);
 int asv_device_count(void);

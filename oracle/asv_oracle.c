@@ -695,12 +695,20 @@ typedef struct {
  *   trace   iters x A x 12 float: per iteration the seed record used by the
  *           flood (x, y, z, L, a, b, m0..m5)
  * Returns A (seeds placed) or a negative value on bad input. */
-ORC_API int orc_run(const orc_params* P, const uint8_t* rgb, const float* fwd, const float* bwd,
-                    int32_t* labels_out, int64_t* sums_out, float* trace)
+/* Plain run from explicit initial seed voxels (gxyz, A x 3) and grid step s
+ * (which sets ratio and the schedule): the seeded form the streaming
+ * blocks use (streamGPU-AS/Render.cpp:434-441 builds kept ++ new seeds).
+ * Outputs as orc_run. */
+ORC_API int orc_run_seeded(const orc_params* P, const uint8_t* rgb, const float* fwd, const float* bwd,
+                           const int* gxyz, int A, int s, int32_t* labels_out, int64_t* sums_out, float* trace)
 {
     int W = P->W, H = P->H, D = P->D;
     size_t V = (size_t)W * H * D;
-    if (W < 1 || H < 1 || D < 1 || P->K < 1 || P->iters < 1) return -1;
+    if (W < 1 || H < 1 || D < 1 || A < 1 || P->iters < 1) return -1;
+    for (int i = 0; i < A; ++i)
+        if (gxyz[i * 3] < 0 || gxyz[i * 3] >= W || gxyz[i * 3 + 1] < 0 || gxyz[i * 3 + 1] >= H ||
+            gxyz[i * 3 + 2] < 0 || gxyz[i * 3 + 2] >= D)
+            return -3;
     uint8_t* prep = NULL;
     const uint8_t* img = rgb;
     if (!P->prepared) {
@@ -715,11 +723,6 @@ ORC_API int orc_run(const orc_params* P, const uint8_t* rgb, const float* fwd, c
     orc_metric_field(fwd, bwd, W, H, D, m6, psd);
     orc_field F = {W, H, D, m6, psd, fwd};
 
-    int s = 0;
-    int A = orc_grid_seeds(W, H, D, P->K, NULL, &s);
-    if (A <= 0) return -2;
-    int* gxyz = (int*)malloc(sizeof(int) * 3 * (size_t)A);
-    orc_grid_seeds(W, H, D, P->K, gxyz, &s);
     if (s < 1) s = 1;
     float ratio = (P->compactness / (float)s) * (P->compactness / (float)s);   /* Image.cpp:1051 */
     int steps[64];
@@ -772,9 +775,23 @@ ORC_API int orc_run(const orc_params* P, const uint8_t* rgb, const float* fwd, c
     if (labels_out) memcpy(labels_out, cur, V * 4);
     if (sums_out) memcpy(sums_out, sums, sizeof(int64_t) * 7 * (size_t)(A + 1));
 
-    free(prep); free(lab); free(m6); free(psd); free(gxyz); free(seeds);
+    free(prep); free(lab); free(m6); free(psd); free(seeds);
     free(L0); free(L1); free(sums);
     return A;
+}
+
+ORC_API int orc_run(const orc_params* P, const uint8_t* rgb, const float* fwd, const float* bwd,
+                    int32_t* labels_out, int64_t* sums_out, float* trace)
+{
+    if (P->W < 1 || P->H < 1 || P->D < 1 || P->K < 1 || P->iters < 1) return -1;
+    int s = 0;
+    int A = orc_grid_seeds(P->W, P->H, P->D, P->K, NULL, &s);
+    if (A <= 0) return -2;
+    int* gxyz = (int*)malloc(sizeof(int) * 3 * (size_t)A);
+    orc_grid_seeds(P->W, P->H, P->D, P->K, gxyz, &s);
+    int r = orc_run_seeded(P, rgb, fwd, bwd, gxyz, A, s, labels_out, sums_out, trace);
+    free(gxyz);
+    return r;
 }
 
 ORC_API int orc_seed_record_floats(void) { return (int)(sizeof(orc_seed) / sizeof(float)); }
