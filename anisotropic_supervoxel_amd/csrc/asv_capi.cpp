@@ -169,6 +169,8 @@ struct asv_ctx {
     SeedRec* rec = nullptr;
     unsigned long long* sums[2] = {nullptr, nullptr};
     float* trace = nullptr;
+    int32_t* map_table = nullptr;
+    int map_cap = 0;
 
     int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
     int depth_cap = 0;      // frames the volume buffers hold
@@ -181,7 +183,7 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace};
+                    c->trace, c->map_table};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -621,6 +623,44 @@ int asv_download_labels(asv_ctx* c, int32_t* labels, void* stream)
     if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
     HIP_TRY(hipMemcpyAsync(labels, c->labels[c->cur], (size_t)c->V() * 4, hipMemcpyDefault, S(stream)));
     HIP_TRY(hipStreamSynchronize(S(stream)));
+    return ASV_OK;
+}
+
+int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table, int32_t n_frames,
+                               int32_t* labels, void* stream)
+{
+    if (!c || !table || !labels || n_table < 1 || n_frames < 0 || n_frames > c->d.D)
+        return fail(ASV_ERR_ARG, "bad label map arguments");
+    if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
+    hipStream_t st = S(stream);
+    if (n_table > c->map_cap) {
+        if (c->map_table) (void)hipFree(c->map_table);
+        c->map_table = nullptr;
+        c->map_cap = 0;
+        HIP_TRY(hipMalloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
+        c->map_cap = n_table;
+    }
+    HIP_TRY(hipMemcpyAsync(c->map_table, table, sizeof(int32_t) * (size_t)n_table, hipMemcpyDefault, st));
+    const int64_t n = (int64_t)n_frames * c->d.W * c->d.H;
+    int32_t* scratch = c->labels[1 - c->cur];          // free after the run
+    HIP_TRY(launch_map_labels(st, c->labels[c->cur], scratch, n, c->map_table, n_table));
+    HIP_TRY(hipMemcpyAsync(labels, scratch, (size_t)n * 4, hipMemcpyDefault, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return ASV_OK;
+}
+
+int asv_host_alloc(int64_t bytes, void** out)
+{
+    if (!out || bytes <= 0) return fail(ASV_ERR_ARG, "bad host allocation request");
+    *out = nullptr;
+    hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(ASV_ERR_NOMEM, "pinned host allocation failed: %s", hipGetErrorString(e));
+    return ASV_OK;
+}
+
+int asv_host_free(void* p)
+{
+    if (p) (void)hipHostFree(p);
     return ASV_OK;
 }
 

@@ -47,4 +47,7 @@ hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long lo
                            unsigned long long* zero_next, const uint32_t* lab, int4* pos,
                            float4* slab);
 
+hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, const int32_t* table,
+                             int n_table);
+
 }  // namespace asv

@@ -1352,4 +1352,25 @@ hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long lo
     return hipGetLastError();
 }
 
+// Label remap through a table (the streaming blocks' block2all map):
+// out[i] = table[in[i]], and 0 for a label outside the table (as an absent
+// std::map key reads in the reference, streamGPU-AS/Render.cpp:559-563).
+__global__ __launch_bounds__(256) void k_map_labels(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                    int64_t n, const int32_t* __restrict__ table, int n_table)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int l = in[i];
+        out[i] = (l >= 0 && l < n_table) ? table[l] : 0;
+    }
+}
+
+hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, const int32_t* table,
+                             int n_table)
+{
+    unsigned blocks = n > 0 ? cdiv(n, 256) : 1;
+    if (blocks > 16384u) blocks = 16384u;
+    hipLaunchKernelGGL(k_map_labels, dim3(blocks), dim3(256), 0, st, in, out, n, table, n_table);
+    return hipGetLastError();
+}
+
 }  // namespace asv

@@ -55,13 +55,46 @@ def _request(r):
     return q
 
 
-def attach_frames(requests):
-    """Generates every stream's frames and flows up front (host memory), so
-    the serving loop reads them as arrived input instead of synthesising."""
-    from .api import synth_video
+class PinnedArray:
+    """A numpy view of page-locked host memory from asv_host_alloc."""
+
+    def __init__(self, shape, dtype):
+        self.lib = _lib.load()
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = ctypes.c_void_p()
+        if self.lib.asv_host_alloc(n, ctypes.byref(p)) != 0:
+            raise _lib.AsvError(self.lib.asv_last_error().decode())
+        self.ptr = p
+        buf = (ctypes.c_char * n).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def __del__(self):
+        try:
+            self.lib.asv_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+def attach_frames(requests, pinned=True):
+    """Generates every stream's frames and flows up front into host memory
+    (page-locked by default), so the serving loop reads them as arrived
+    input instead of synthesising them."""
+    lib = _lib.load()
     for r in requests:
-        rgb, fwd, bwd, _ = synth_video(r["video_seed"], r["width"], r["height"], r["frames"], r["n_ellipses"],
-                                       r["vmax"], r["noise_sigma"])
+        D, H, W = r["frames"], r["height"], r["width"]
+        if pinned:
+            bufs = [PinnedArray((D, H, W, 3), np.uint8), PinnedArray((D, H, W, 2), np.float32),
+                    PinnedArray((D, H, W, 2), np.float32)]
+            rgb, fwd, bwd = (b.array for b in bufs)
+            r["_pinned"] = bufs
+        else:
+            rgb = np.empty((D, H, W, 3), np.uint8)
+            fwd = np.empty((D, H, W, 2), np.float32)
+            bwd = np.empty((D, H, W, 2), np.float32)
+        rc = lib.asv_synth_video(r["video_seed"], W, H, D, r["n_ellipses"], r["vmax"], r["noise_sigma"],
+                                 rgb.ctypes.data, fwd.ctypes.data, bwd.ctypes.data, None)
+        if rc != 0:
+            raise _lib.AsvError("synthetic video failed")
         r["frames_data"] = (rgb, fwd, bwd)
     return requests
 

@@ -148,6 +148,19 @@ int asv_download_labels(asv_ctx* ctx, int32_t* labels, void* stream);
  * slot texels read back at GPU-AS/Image.cpp:1585-1588. */
 int asv_download_sums(asv_ctx* ctx, int64_t* sums, void* stream);
 
+/* The first n_frames frames of the final labels mapped through table
+ * (label l -> table[l], labels outside [0, n_table) -> 0) on the device,
+ * then copied out: the block2all stitch of the streaming program
+ * (streamGPU-AS/Render.cpp:559-563) without a full label download.
+ * Synchronises stream. */
+int asv_download_labels_mapped(asv_ctx* ctx, const int32_t* table, int32_t n_table, int32_t n_frames,
+                               int32_t* labels, void* stream);
+
+/* Page-locked host memory for frame and label buffers (faster transfers).
+ * No counterpart in the reference; for callers that stage frames. */
+int asv_host_alloc(int64_t bytes, void** out);
+int asv_host_free(void* p);
+
 /* Per-iteration seed records, when enabled before the run:
  * lloyd_iters x A x 12 float (x, y, z, L, a, b, m0..m5). */
 int asv_enable_trace(asv_ctx* ctx, int32_t on);
