@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="plain-640x480x200-K10000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=12)
+    ap.add_argument("--cpu-frames", type=int, default=70)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB torch copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
