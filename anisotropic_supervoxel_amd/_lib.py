@@ -3,6 +3,10 @@
 The library is built in-tree (``anisotropic_supervoxel_amd/libasv.so``) by
 ``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
 is no CPU fallback for any entry point.
+
+A process that also uses torch's GPU support imports torch first: torch's
+bundled HIP runtime and /opt/rocm's share a soname, the first one loaded
+serves the process, and torch only initialises on its own.
 """
 import ctypes
 import os
@@ -53,6 +57,8 @@ SIGNATURES = {
     "asv_eval_metrics": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "asv_plan_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
     "asv_download_labels_mapped": (ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _vp]),
+    "asv_postprocess_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "asv_postprocess_labels": (ctypes.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
     "asv_host_alloc": (ctypes.c_int, [_i64, ctypes.POINTER(_vp)]),
     "asv_host_free": (ctypes.c_int, [_vp]),
     "asv_reseed": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp]),

@@ -171,6 +171,8 @@ struct asv_ctx {
     float* trace = nullptr;
     int32_t* map_table = nullptr;
     int map_cap = 0;
+    long long* post_scratch = nullptr;
+    int64_t post_cap = 0;
 
     int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
     int depth_cap = 0;      // frames the volume buffers hold
@@ -183,7 +185,7 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace, c->map_table};
+                    c->trace, c->map_table, c->post_scratch};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -646,6 +648,128 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
     HIP_TRY(launch_map_labels(st, c->labels[c->cur], scratch, n, c->map_table, n_table));
     HIP_TRY(hipMemcpyAsync(labels, scratch, (size_t)n * 4, hipMemcpyDefault, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return ASV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The decision of check_dead_seeds (GPU-AS/Image.cpp:1579-1720) from the
+// per-slot counts: which rewrite the labels need and its table.
+//   mode -1: none (no dead seed, no unlabelled voxel);
+//   mode  0: label_replace composition for up to three dead seeds (the
+//            current last id takes the dead id, once per dead seed, in
+//            order), then, if unlabelled voxels exist, the raster fill;
+//   mode  1: more than three dead seeds: alive ids to their rank.
+struct PostPlan {
+    int mode = -1;
+    std::vector<int32_t> table;
+    int dead = 0, valid = 0;
+};
+
+PostPlan post_plan(const int64_t* sums, int A)
+{
+    PostPlan pl;
+    const bool invalid = sums[(size_t)A * 8 + 3] > 0;
+    std::vector<int> dead;
+    std::vector<int32_t> rank((size_t)A, 0);
+    for (int i = 0; i < A; ++i) {
+        if (sums[(size_t)i * 8 + 3] < 1) dead.push_back(i);
+        else rank[(size_t)i] = pl.valid++;
+    }
+    pl.dead = (int)dead.size();
+    if (pl.dead == 0) {
+        if (invalid) pl.mode = 0;
+    } else if (pl.dead <= 3) {
+        pl.mode = 0;
+        pl.table.resize((size_t)A);
+        for (int l = 0; l < A; ++l) {
+            int v = l, last = A;
+            for (int d : dead) {
+                last -= 1;
+                if (v == last) v = d;
+            }
+            pl.table[(size_t)l] = v;
+        }
+    } else {
+        pl.mode = 1;
+        pl.table.swap(rank);
+    }
+    return pl;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asv_postprocess_device(asv_ctx* c, const int64_t* sums, int32_t* dead_out, int32_t* final_label_num,
+                           void* stream)
+{
+    if (!c || !sums) return fail(ASV_ERR_ARG, "null argument");
+    if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
+    const int A = c->d.A;
+    PostPlan pl = post_plan(sums, A);
+    if (dead_out) *dead_out = pl.dead;
+    if (final_label_num) *final_label_num = pl.valid;
+    if (pl.mode < 0) return ASV_OK;
+    hipStream_t st = S(stream);
+    const int64_t V = c->V();
+    const int64_t words = post_scratch_words(V);
+    if (words > c->post_cap) {
+        if (c->post_scratch) (void)hipFree(c->post_scratch);
+        c->post_scratch = nullptr;
+        c->post_cap = 0;
+        HIP_TRY(hipMalloc((void**)&c->post_scratch, sizeof(long long) * (size_t)words));
+        c->post_cap = words;
+    }
+    const int n_table = (int)pl.table.size();
+    if (n_table > c->map_cap) {
+        if (c->map_table) (void)hipFree(c->map_table);
+        c->map_table = nullptr;
+        c->map_cap = 0;
+        HIP_TRY(hipMalloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
+        c->map_cap = n_table;
+    }
+    if (n_table > 0)
+        HIP_TRY(hipMemcpyAsync(c->map_table, pl.table.data(), sizeof(int32_t) * (size_t)n_table, hipMemcpyDefault,
+                               st));
+    HIP_TRY(launch_postprocess(st, c->labels[c->cur], c->labels[1 - c->cur], V, pl.mode, c->map_table, n_table,
+                               c->post_scratch));
+    c->cur = 1 - c->cur;
+    HIP_TRY(hipStreamSynchronize(st));          // the table's host copy is local
+    return ASV_OK;
+}
+
+int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
+                           int32_t* dead_out, int32_t* final_label_num, void* stream)
+{
+    if (!sums || !in || !out || A < 1 || V < 1 || in == out) return fail(ASV_ERR_ARG, "bad post-process arguments");
+    PostPlan pl = post_plan(sums, A);
+    if (dead_out) *dead_out = pl.dead;
+    if (final_label_num) *final_label_num = pl.valid;
+    hipStream_t st = S(stream);
+    if (pl.mode < 0) {
+        HIP_TRY(hipMemcpyAsync(out, in, sizeof(int32_t) * (size_t)V, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return ASV_OK;
+    }
+    const int64_t words = post_scratch_words(V);
+    long long* scratch = nullptr;
+    int32_t* table = nullptr;
+    const int n_table = (int)pl.table.size();
+    HIP_TRY(hipMalloc((void**)&scratch, sizeof(long long) * (size_t)words));
+    hipError_t e = hipSuccess;
+    if (n_table > 0) {
+        e = hipMalloc((void**)&table, sizeof(int32_t) * (size_t)n_table);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(table, pl.table.data(), sizeof(int32_t) * (size_t)n_table, hipMemcpyDefault, st);
+    }
+    if (e == hipSuccess) e = launch_postprocess(st, in, out, V, pl.mode, table, n_table, scratch);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(scratch);
+    if (table) (void)hipFree(table);
+    if (e != hipSuccess) return fail(ASV_ERR_HIP, "post-process: %s", hipGetErrorString(e));
     return ASV_OK;
 }
 

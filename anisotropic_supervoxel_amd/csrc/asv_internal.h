@@ -50,4 +50,12 @@ hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long lo
 hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, const int32_t* table,
                              int n_table);
 
+// Device post-processing: mode 0 fills unlabelled voxels from the previous
+// labelled voxel after mapping through `table` (label_replace composition),
+// mode 1 is the dead-seed rank remap.  `scratch` holds post_scratch_words(V)
+// 64-bit words.  out must not alias in.
+int64_t post_scratch_words(int64_t V);
+hipError_t launch_postprocess(hipStream_t st, const int32_t* in, int32_t* out, int64_t V, int mode,
+                              const int32_t* table, int n_table, long long* scratch);
+
 }  // namespace asv

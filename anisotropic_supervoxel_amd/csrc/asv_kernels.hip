@@ -1373,4 +1373,144 @@ hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, in
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// post-processing of the final labels on the device
+// (check_dead_seeds / label_replace.gs / sequence_label, GPU-AS/Image.cpp:1579-1778)
+// ---------------------------------------------------------------------------
+//
+// Both raster-order rules of sequence_label carry "the last voxel so far
+// that satisfies a predicate", i.e. a prefix maximum of
+// j(i) = max{ j <= i : pred(j) } over the linear volume:
+//   kPostFill  (no dead-seed rewrite): pred(j) = in[j] != -1; out[i] is
+//     m(in[j(i)]) with m the label_replace composition table (identity when
+//     no seed died); a voxel with no such j reads 0, except voxel 0, which
+//     keeps its -1 (:1750-1776);
+//   kPostRemap (more than three dead seeds): pred(j) = j == 0 or in[j] is
+//     neither in[0] nor -1; out[i] = seq(in[j(i)]) with seq the alive rank
+//     (0 for any other id, as std::map::operator[] gives) (:1727-1748).
+// Three kernels: per-chunk maximum, one-block exclusive scan over chunks,
+// and the apply pass that finishes the scan inside each chunk.
+constexpr int kPostPer = 16;                       // voxels per thread
+constexpr int kPostChunk = 256 * kPostPer;         // voxels per block
+
+__device__ __forceinline__ bool post_pred(int mode, int64_t i, int l, int l0)
+{
+    return mode == 0 ? l != -1 : (i == 0 || (l != l0 && l != -1));
+}
+
+__device__ __forceinline__ int post_value(int mode, int l, const int32_t* __restrict__ tab, int n_table)
+{
+    if (l >= 0 && l < n_table) return tab[l];
+    return mode == 0 ? l : 0;
+}
+
+__global__ __launch_bounds__(256) void k_post_chunk_max(const int32_t* __restrict__ in, int64_t V, int mode,
+                                                        long long* __restrict__ chunk_max)
+{
+    __shared__ long long s_w[4];
+    const int l0 = in[0];
+    const int64_t base = (int64_t)blockIdx.x * kPostChunk + (int64_t)threadIdx.x * kPostPer;
+    long long m = -1;
+    for (int e = 0; e < kPostPer; ++e) {
+        const int64_t i = base + e;
+        if (i < V && post_pred(mode, i, in[i], l0)) m = i;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const long long o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long r = s_w[0];
+        for (int w = 1; w < 4; ++w) r = s_w[w] > r ? s_w[w] : r;
+        chunk_max[blockIdx.x] = r;
+    }
+}
+
+// exclusive prefix maximum over n chunk maxima, in place, one block
+__global__ __launch_bounds__(1024) void k_post_scan(long long* __restrict__ v, int64_t n)
+{
+    __shared__ long long s_t[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b = (int64_t)t * per, e = b + per < n ? b + per : n;
+    long long m = -1;
+    for (int64_t i = b; i < e; ++i) m = v[i] > m ? v[i] : m;
+    s_t[t] = m;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long o = t >= off ? s_t[t - off] : -1;
+        __syncthreads();
+        s_t[t] = o > s_t[t] ? o : s_t[t];
+        __syncthreads();
+    }
+    long long run = t > 0 ? s_t[t - 1] : -1;
+    for (int64_t i = b; i < e; ++i) {
+        const long long x = v[i];
+        v[i] = run;
+        run = x > run ? x : run;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_post_apply(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                    int64_t V, int mode, const int32_t* __restrict__ tab,
+                                                    int n_table, const long long* __restrict__ chunk_prefix)
+{
+    __shared__ long long s_w[4];
+    const int l0 = in[0];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kPostChunk + (int64_t)t * kPostPer;
+    int l[kPostPer];
+    long long m = -1;
+#pragma unroll
+    for (int e = 0; e < kPostPer; ++e) {
+        const int64_t i = base + e;
+        l[e] = i < V ? in[i] : -1;
+        if (i < V && post_pred(mode, i, l[e], l0)) m = i;
+    }
+    // exclusive prefix maximum of the per-thread maxima over the block
+    long long inc = m;
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long o = __shfl_up(inc, off);
+        if (lane >= off) inc = o > inc ? o : inc;
+    }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    long long carry = chunk_prefix[blockIdx.x];
+    for (int w = 0; w < wave; ++w) carry = s_w[w] > carry ? s_w[w] : carry;
+    long long excl = __shfl_up(inc, 1);
+    if (lane == 0) excl = -1;
+    long long run = excl > carry ? excl : carry;
+#pragma unroll
+    for (int e = 0; e < kPostPer; ++e) {
+        const int64_t i = base + e;
+        if (i >= V) break;
+        int r;
+        if (post_pred(mode, i, l[e], l0)) {
+            run = i;
+            r = post_value(mode, l[e], tab, n_table);
+        } else if (run >= 0) {
+            r = post_value(mode, in[run], tab, n_table);
+        } else {
+            r = i == 0 ? l[e] : 0;
+        }
+        out[i] = r;
+    }
+}
+
+int64_t post_scratch_words(int64_t V) { return V > 0 ? (V + kPostChunk - 1) / kPostChunk : 1; }
+
+hipError_t launch_postprocess(hipStream_t st, const int32_t* in, int32_t* out, int64_t V, int mode,
+                              const int32_t* table, int n_table, long long* scratch)
+{
+    const int64_t nch = post_scratch_words(V);
+    if (nch > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_post_chunk_max, dim3((unsigned)nch), dim3(256), 0, st, in, V, mode, scratch);
+    hipLaunchKernelGGL(k_post_scan, dim3(1), dim3(1024), 0, st, scratch, nch);
+    hipLaunchKernelGGL(k_post_apply, dim3((unsigned)nch), dim3(256), 0, st, in, out, V, mode, table, n_table,
+                       (const long long*)scratch);
+    return hipGetLastError();
+}
+
 }  // namespace asv

@@ -241,7 +241,6 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
     }
     if (rc) return rc;
     asv_ctx* ctx = s->ctx;
-    const size_t Vb = plane * Db;
     std::vector<int64_t> sums((size_t)(A + 1) * 8);
     rc = asv_upload_frames_range(ctx, resident, Db - resident, rgb + (size_t)resident * plane * 3,
                                  fwd + (size_t)resident * plane * 2, bwd + (size_t)resident * plane * 2, stream);
@@ -252,27 +251,13 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
     const int done_frames = last ? Db : Db - O;
     int32_t* dst = labels_all + (size_t)start * plane;
 
-    // check_dead_seeds (streamGPU-AS/Image.cpp:1534-1579).  Without dead
-    // seeds or unlabelled voxels it leaves the labels as they are, and the
-    // block2all stitch (:559-563) runs on the device; otherwise the labels
-    // come back, are post-processed and stitched on the host.
-    bool clean = sums[(size_t)A * 8 + 3] == 0;
-    for (int i = 0; i < A && clean; ++i) clean = sums[(size_t)i * 8 + 3] >= 1;
+    // check_dead_seeds (streamGPU-AS/Image.cpp:1534-1579) on the device, then
+    // the block2all stitch (:559-563), where an id absent from the map reads
+    // as 0 (std::map operator[]); only the stitched frames come back.
     int dead = 0, nfin = 0;
-    if (clean) {
-        rc = asv_download_labels_mapped(ctx, block2all.data(), A, done_frames, dst, stream);
-        if (rc) return rc;
-    } else {
-        std::vector<int32_t> labels(Vb);
-        rc = asv_download_labels(ctx, labels.data(), stream);
-        if (!rc) rc = asv_postprocess(sums.data(), A, labels.data(), (int64_t)Vb, &dead, &nfin);
-        if (rc) return rc;
-        // an id absent from the map reads as 0 (std::map operator[])
-        for (size_t i = 0; i < (size_t)done_frames * plane; ++i) {
-            const int l = labels[i];
-            dst[i] = (l >= 0 && l < A) ? block2all[(size_t)l] : 0;
-        }
-    }
+    rc = asv_postprocess_device(ctx, sums.data(), &dead, &nfin, stream);
+    if (!rc) rc = asv_download_labels_mapped(ctx, block2all.data(), A, done_frames, dst, stream);
+    if (rc) return rc;
     std::vector<KeptSeed> alive;
     for (int i = 0; i < A; ++i) {
         const int64_t* q = &sums[(size_t)i * 8];

@@ -156,6 +156,20 @@ int asv_download_sums(asv_ctx* ctx, int64_t* sums, void* stream);
 int asv_download_labels_mapped(asv_ctx* ctx, const int32_t* table, int32_t n_table, int32_t n_frames,
                                int32_t* labels, void* stream);
 
+/* check_dead_seeds (GPU-AS/Image.cpp:1579-1720) on the device: from the
+ * final counts in sums ((A+1) x 8 as asv_download_sums writes them), the
+ * labels of the context are rewritten as the reference's label_replace
+ * (label_replace.gs:18-34) and sequence_label (:1727-1778) leave them;
+ * later downloads read the result.  Outputs the dead-seed count and the
+ * final label count (final_label_num).  Synchronises stream. */
+int asv_postprocess_device(asv_ctx* ctx, const int64_t* sums, int32_t* dead_out, int32_t* final_label_num,
+                           void* stream);
+
+/* The same rewrite between two device buffers of V labels (in != out),
+ * GL raster order, for a label volume produced elsewhere.  Synchronises. */
+int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
+                           int32_t* dead_out, int32_t* final_label_num, void* stream);
+
 /* Page-locked host memory for frame and label buffers (faster transfers).
  * No counterpart in the reference; for callers that stage frames. */
 int asv_host_alloc(int64_t bytes, void** out);

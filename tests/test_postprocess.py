@@ -85,3 +85,65 @@ def test_eval_metrics_closed_forms():
     # region 0 (64 voxels) meets supervoxel 0 (80): 16/64; region 1 meets
     # supervoxels 0 and 1 (80 + 48): 64/64
     assert ue == pytest.approx((16 / 64 + 64 / 64) / 2)
+
+
+def _device_post(sums, A, labels):
+    import ctypes
+    import torch
+    lib = asv._lib.load()
+    d_in = torch.from_numpy(np.ascontiguousarray(labels, np.int32)).cuda()
+    d_out = torch.full_like(d_in, -7)
+    torch.cuda.synchronize()
+    dead = ctypes.c_int32(-1)
+    fin = ctypes.c_int32(-1)
+    s = np.ascontiguousarray(sums, np.int64)
+    asv._lib.check(lib.asv_postprocess_labels(s.ctypes.data, A, d_in.data_ptr(), d_out.data_ptr(), labels.size,
+                                              ctypes.byref(dead), ctypes.byref(fin), None))
+    return d_out.cpu().numpy(), dead.value, fin.value
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("A,dead,unl,lead", [
+    (12, [], 7, False), (12, [], 7, True), (12, [3], 0, False), (12, [0, 11], 5, False),
+    (12, [2, 5, 9], 3, False), (12, [1, 2, 4, 8, 10], 0, False), (12, [0, 3, 4, 6], 6, True),
+])
+def test_device_postprocess_matches_restatement(have_gpu, A, dead, unl, lead):
+    rng = np.random.default_rng(A + len(dead) * 7 + unl)
+    labels, sums, counts = _case(rng, A, 400, dead, unl, lead)
+    got, ndead, nfin = _device_post(sums, A, labels)
+    want, wdead, wfin = ref.check_dead_seeds(list(counts), A, list(labels))
+    assert (ndead, nfin) == (wdead, wfin)
+    assert got.tolist() == want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["fill", "fill_lead_run", "replace_fill", "remap", "remap_lead_run", "remap_l0_runs"])
+def test_device_postprocess_large_matches_host(have_gpu, case):
+    """Volumes spanning many 4096-voxel scan chunks, with runs of -1 (and of
+    the first voxel's label) that cross chunk boundaries."""
+    rng = np.random.default_rng(sum(map(ord, case)))
+    A, V = 300, 4096 * 29 + 1234
+    dead = {"fill": [], "fill_lead_run": [], "replace_fill": [7, 299], "remap": [1, 5, 9, 13, 200],
+            "remap_lead_run": [2, 3, 4, 5], "remap_l0_runs": [10, 11, 12, 13, 14]}[case]
+    alive = np.array([i for i in range(A) if i not in dead])
+    labels = np.repeat(rng.choice(alive, V // 37 + 1), 37)[:V].astype(np.int32)
+    if case != "remap":
+        for _ in range(6):                     # long -1 runs across chunk edges
+            a = int(rng.integers(0, V - 9000))
+            labels[a:a + int(rng.integers(1, 9000))] = -1
+    if case.endswith("lead_run"):
+        labels[:10000] = -1
+    if case == "remap_l0_runs":
+        l0 = labels[0]
+        for _ in range(5):
+            a = int(rng.integers(1, V - 6000))
+            labels[a:a + 6000] = l0
+    counts = np.array([(labels == i).sum() for i in range(A)] + [(labels < 0).sum()], np.int64)
+    sums = np.zeros((A + 1, 8), np.int64)
+    sums[:, 3] = counts
+    got, ndead, nfin = _device_post(sums, A, labels)
+    want, wdead, wfin = api.postprocess(sums, A, labels)
+    nd = int((counts[:A] < 1).sum())          # planted runs can empty more ids
+    assert nd >= len(dead)
+    assert (ndead, nfin) == (wdead, wfin) == (nd, A - nd)
+    assert np.array_equal(got, want)
