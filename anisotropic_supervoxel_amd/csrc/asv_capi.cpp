@@ -3,6 +3,7 @@
 // round trip inside the Lloyd loop.
 #include "../../include/asv.h"
 #include "asv_internal.h"
+#include "asv_host_internal.h"
 
 #include <cmath>
 #include <cstdio>
@@ -372,6 +373,20 @@ int asv_destroy(asv_ctx* c)
     delete c;
     return ASV_OK;
 }
+
+}  // extern "C"
+
+bool asv::ctx_reusable(const asv_ctx* c, const asv_params* p, int32_t seeds)
+{
+    return c && p && c->p.width == p->width && c->p.height == p->height && c->depth_cap >= p->depth &&
+           c->seed_cap >= seeds && c->p.lloyd_iters == p->lloyd_iters && c->p.compactness == p->compactness &&
+           c->p.tie_rule == p->tie_rule && c->p.angle_gate == p->angle_gate && c->p.schedule == p->schedule &&
+           c->p.frames_prepared == p->frames_prepared && !c->trace_on;
+}
+
+int32_t asv::ctx_seed_cap(const asv_ctx* c) { return c ? c->seed_cap : 0; }
+
+extern "C" {
 
 int asv_seeds_placed(const asv_ctx* c) { return c ? c->d.A : fail(ASV_ERR_ARG, "null context"); }
 int asv_grid_step(const asv_ctx* c) { return c ? c->s : fail(ASV_ERR_ARG, "null context"); }

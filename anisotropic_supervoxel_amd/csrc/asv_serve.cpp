@@ -11,6 +11,7 @@
 // when it became ready (frames present and the previous window done), when
 // its GPU work was enqueued and when its labels were back.
 #include "../../include/asv.h"
+#include "asv_host_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -111,6 +112,15 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     std::deque<int> work;            // admitted request indices
     int active = 0;
     int64_t active_bytes = 0;
+    // Idle resident contexts left by finished streams, oldest first.  They
+    // count against the memory budget and are freed when admission needs
+    // the room or more than max_active of them wait.
+    struct Idle {
+        asv_ctx* c;
+        int64_t bytes;
+    };
+    std::deque<Idle> idle;
+    int64_t idle_bytes = 0;
     bool closing = false;
     int nwin = 0;
     const auto t0 = clock_t_::now();
@@ -121,12 +131,23 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         ++nwin;
     };
 
+    // A new stream adopts an idle context of its frame size and settings
+    // when one is there (no device allocation on admission).
     auto process = [&](int ri, hipStream_t st) -> int {
         const asv_serve_request& r = reqs[ri];
         asv_stream_params p = stream_params_of(sp, &r);
         asv_stream* s = nullptr;
         int rc = asv_stream_create(&p, &s);
         if (rc) return rc;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            for (auto it = idle.begin(); it != idle.end(); ++it)
+                if (asv::stream_adopt_ctx(s, it->c)) {
+                    idle_bytes -= it->bytes;
+                    idle.erase(it);
+                    break;
+                }
+        }
         const int nb = asv_stream_block_count(s);
         std::vector<int32_t> iv((size_t)nb * 2);
         asv_stream_plan(&p, iv.data(), nb);
@@ -179,7 +200,22 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             results[ri].labels_hash = fnv1a(labels.data(), labels.size(), 1469598103934665603ull);
             results[ri].voxel_iters = (int64_t)vox_iters;
         }
+        asv_ctx* left = rc ? nullptr : asv::stream_take_ctx(s);
         asv_stream_destroy(s);
+        if (left) {
+            asv_ctx* evict = nullptr;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                idle.push_back(Idle{left, asv_device_bytes(left)});
+                idle_bytes += idle.back().bytes;
+                if ((int)idle.size() > sp->max_active) {
+                    evict = idle.front().c;
+                    idle_bytes -= idle.front().bytes;
+                    idle.pop_front();
+                }
+            }
+            if (evict) asv_destroy(evict);
+        }
         return rc;
     };
 
@@ -233,7 +269,19 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
                     it = pending.erase(it);
                     continue;
                 }
-                if (active < sp->max_active && active_bytes + fp <= sp->memory_budget) {
+                if (active < sp->max_active && active_bytes + fp <= sp->memory_budget &&
+                    active_bytes + idle_bytes + fp > sp->memory_budget && !idle.empty()) {
+                    // room is held by idle contexts: free the oldest, then rescan
+                    asv_ctx* evict = idle.front().c;
+                    idle_bytes -= idle.front().bytes;
+                    idle.pop_front();
+                    lk.unlock();
+                    asv_destroy(evict);
+                    lk.lock();
+                    it = pending.begin();
+                    continue;
+                }
+                if (active < sp->max_active && active_bytes + idle_bytes + fp <= sp->memory_budget) {
                     ++active;
                     active_bytes += fp;
                     results[ri].footprint = fp;
@@ -257,6 +305,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     }
     cv.notify_all();
     for (auto& th : pool) th.join();
+    for (const Idle& e : idle) asv_destroy(e.c);
     if (n_windows) *n_windows = nwin;
     if (wall_ms) *wall_ms = ms_since(t0);
     for (int i = 0; i < n; ++i)

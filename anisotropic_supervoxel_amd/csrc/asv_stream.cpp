@@ -5,6 +5,7 @@
 // per-block maps from block seed ids to global ids.  Host logic over the C
 // ABI; the device work of a block is asv_create_seeded + asv_run.
 #include "../../include/asv.h"
+#include "asv_host_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -95,6 +96,45 @@ struct asv_stream {
         if (ctx) asv_destroy(ctx);
     }
 };
+
+namespace {
+
+// Parameters of the stream's resident context (all blocks share it).
+asv_params ctx_params(const asv_stream* s, int seeds)
+{
+    asv_params cp;
+    asv_default_params(&cp, s->p.width, s->p.height, s->depth_cap, seeds);
+    cp.lloyd_iters = s->p.lloyd_iters;
+    cp.compactness = s->p.compactness;
+    cp.angle_gate = s->p.angle_gate;
+    cp.tie_rule = s->p.tie_rule;
+    cp.frames_prepared = s->p.frames_prepared;
+    return cp;
+}
+
+// Seed capacity a new resident context gets: four blocks' budget.
+int wanted_seed_cap(const asv_stream* s) { return 4 * (s->seeds100 * s->p.block_size / 100) + 64; }
+
+}  // namespace
+
+asv_ctx* asv::stream_take_ctx(asv_stream* s)
+{
+    if (!s) return nullptr;
+    asv_ctx* c = s->ctx;
+    s->ctx = nullptr;
+    s->seed_cap = 0;
+    return c;
+}
+
+bool asv::stream_adopt_ctx(asv_stream* s, asv_ctx* c)
+{
+    if (!c || !s || s->ctx || s->next_block != 0) return false;
+    const asv_params cp = ctx_params(s, wanted_seed_cap(s));
+    if (!asv::ctx_reusable(c, &cp, wanted_seed_cap(s))) return false;
+    s->ctx = c;
+    s->seed_cap = asv::ctx_seed_cap(c);
+    return true;
+}
 
 extern "C" {
 
@@ -226,16 +266,9 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
     } else {
         if (s->ctx) asv_destroy(s->ctx);
         s->ctx = nullptr;
-        asv_params cp;
-        asv_default_params(&cp, W, H, s->depth_cap, budget);
-        cp.lloyd_iters = s->p.lloyd_iters;
-        cp.compactness = s->p.compactness;
-        cp.angle_gate = s->p.angle_gate;
-        cp.tie_rule = s->p.tie_rule;
-        cp.frames_prepared = s->p.frames_prepared;
-        s->seed_cap = 4 * (s->seeds100 * s->p.block_size / 100) + 64;
+        s->seed_cap = wanted_seed_cap(s);
         if (s->seed_cap < 2 * A) s->seed_cap = 2 * A;
-        cp.seeds_num = s->seed_cap;
+        asv_params cp = ctx_params(s, s->seed_cap);
         rc = asv_create_seeded(&cp, xyz.data(), A, sgrid, &s->ctx);
         if (!rc) rc = asv_reseed(s->ctx, xyz.data(), A, sgrid, Db, stream);
     }

@@ -74,3 +74,18 @@ def test_admission_limits_concurrency(have_gpu):
     iv = sorted(spans.values())
     for (a0, a1), (b0, b1) in zip(iv, iv[1:]):
         assert b0 >= a1 - 1e-6
+
+
+@pytest.mark.gpu
+def test_reused_context_equals_fresh(have_gpu):
+    """Streams of one frame size through one worker: each later stream adopts
+    the previous one's resident context when its seed and frame capacity
+    suffice (different lengths and densities exercise both outcomes)."""
+    spec = [(33, 2.0), (21, 1.0), (40, 3.0), (26, 2.0), (17, 2.5)]
+    reqs = [dict(stream_id=i, width=48, height=40, frames=D, density=d, n_ellipses=5, video_seed=200 + i,
+                 vmax=2.0, noise_sigma=4.0, arrival_ms=0.0) for i, (D, d) in enumerate(spec)]
+    seq, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), reqs)
+    assert all(r["status"] == 0 for r in seq)
+    for i, r in enumerate(reqs):
+        alone, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), [r])
+        assert alone[0]["labels_hash"] == seq[i]["labels_hash"], r
