@@ -13,7 +13,7 @@
 //   k_seed_metric         Image.cpp:689-872 and :1351-1534
 //   k_fill + k_place      draw_seeds / draw_next_seeds (clear -1, later id wins)
 //   k_jfa                 shaders/flood.gs:53-209
-//   k_reduce              shaders/centroids.vs:19-53 with additive blending
+//   k_reduce_bricks       shaders/centroids.vs:19-53 with additive blending
 //   k_finalize            shaders/aver_exception.vs:22-81
 
 #include "asv_internal.h"
@@ -976,25 +976,28 @@ __global__ __launch_bounds__(256) void k_jfa_direct(int W, int H, int D, int A, 
 // centroid reduction (centroids.vs:19-53)
 // ---------------------------------------------------------------------------
 
-// Each thread walks 16 consecutive voxels of the linear volume, keeping a
-// run of equal labels in registers; for a run of n voxels from x = a to
-// x = e on row (y, z) the position sums are closed forms:
+// Bricks of 64 columns x ROWS rows x zc frames: each thread walks SEG
+// consecutive voxels of one row per frame, keeping a run of equal labels in
+// registers; for a run of n voxels from x = a to x = e on row (y, z) the
+// position sums are closed forms
 //   sum(2x+1) = (e+1)^2 - a^2,  sum(2y+1) = n(2y+1),  sum(2z+1) = n(2z+1),
 // and only L, a, b are accumulated.  A finished run is added into the
-// block's LDS table (open addressing by label, 64-bit LDS atomics); the
-// table is flushed to the global slots with one 64-bit atomic per sum and
-// label at the end.  Integer sums are exact, so the result does not depend
-// on the order of any of these additions.
-constexpr int kSeg = 16;                 // voxels per thread per chunk
-constexpr int kRedChunks = 4;            // chunks per block: 256*16*4 voxels
+// block's LDS table (open addressing by label, at most kRedProbes probes,
+// 64-bit LDS atomics), or straight into the global slot when its probes are
+// taken; the table goes to the global slots with one 64-bit atomic per sum
+// and label at the end.  Compact bricks keep the labels a block meets few,
+// and later frames of a brick mostly hit the slots of earlier ones.
+// Integer sums are exact, so no order of these additions changes a result.
 constexpr int kRedSlots = 256;
+constexpr int kRedProbes = 16;
+constexpr int kRedTileW = 64;
 
 __device__ __forceinline__ void run_flush(int label, int A, int xa, int xe, int y, int z, unsigned sl,
                                           unsigned sa, unsigned sb, int* s_key,
                                           unsigned long long (*s_acc)[8], unsigned long long* __restrict__ sums)
 {
-    int key = label < 0 ? A : label;
-    unsigned long long n = (unsigned long long)(xe - xa + 1);
+    const int key = label < 0 ? A : label;
+    const unsigned long long n = (unsigned long long)(xe - xa + 1);
     unsigned long long vals[7];
     vals[0] = (unsigned long long)(xe + 1) * (unsigned long long)(xe + 1) - (unsigned long long)xa * (unsigned long long)xa;
     vals[1] = n * (unsigned long long)(2 * y + 1);
@@ -1003,10 +1006,9 @@ __device__ __forceinline__ void run_flush(int label, int A, int xa, int xe, int 
     vals[4] = sl;
     vals[5] = sa;
     vals[6] = sb;
-    unsigned h = hash_slot(key) << 1;           // 8-bit start slot
-    h &= kRedSlots - 1;
-    for (int p = 0; p < kRedSlots; ++p) {
-        int old = atomicCAS(&s_key[h], kEmpty, key);
+    unsigned h = (hash_slot(key) << 1) & (kRedSlots - 1);
+    for (int p = 0; p < kRedProbes; ++p) {
+        const int old = atomicCAS(&s_key[h], kEmpty, key);
         if (old == kEmpty || old == key) {
 #pragma unroll
             for (int i = 0; i < 7; ++i) atomicAdd(&s_acc[h][i], vals[i]);
@@ -1014,17 +1016,19 @@ __device__ __forceinline__ void run_flush(int label, int A, int xa, int xe, int 
         }
         h = (h + 1) & (kRedSlots - 1);
     }
-    // table full: straight to the global slot
     unsigned long long* o = sums + (size_t)key * kSumWords;
 #pragma unroll
     for (int i = 0; i < 7; ++i) atomicAdd(o + i, vals[i]);
 }
 
-__global__ __launch_bounds__(256) void k_reduce_runs(int W, int H, int64_t V, int A,
-                                                     const int32_t* __restrict__ labels,
-                                                     const uint32_t* __restrict__ lab,
-                                                     unsigned long long* __restrict__ sums)
+template <int SEG>
+__global__ __launch_bounds__(256) void k_reduce_bricks(int W, int H, int D, int A, int tiles_x, int tiles_y, int zc,
+                                                       const int32_t* __restrict__ labels,
+                                                       const uint32_t* __restrict__ lab,
+                                                       unsigned long long* __restrict__ sums)
 {
+    constexpr int TPR = kRedTileW / SEG;           // threads per row
+    constexpr int ROWS = 256 / TPR;
     __shared__ int s_key[kRedSlots];
     __shared__ unsigned long long s_acc[kRedSlots][8];
     const int tid = threadIdx.x;
@@ -1033,120 +1037,62 @@ __global__ __launch_bounds__(256) void k_reduce_runs(int W, int H, int64_t V, in
     for (int i = 0; i < 8; ++i) s_acc[tid][i] = 0ull;
     __syncthreads();
 
-    const int64_t block_base = (int64_t)blockIdx.x * (256 * kSeg * kRedChunks);
-    for (int ch = 0; ch < kRedChunks; ++ch) {
-        const int64_t v0 = block_base + ((int64_t)ch * 256 + tid) * kSeg;
-        if (v0 >= V) break;
-        int lv[kSeg];
-        uint32_t cv[kSeg];
-        if (v0 + kSeg <= V) {
-            const int4* lp = reinterpret_cast<const int4*>(labels + v0);
-            const uint4* cp = reinterpret_cast<const uint4*>(lab + v0);
+    const int b = blockIdx.x;
+    const int tx = b % tiles_x, ty = (b / tiles_x) % tiles_y, tz = b / (tiles_x * tiles_y);
+    const int xs = tx * kRedTileW + (tid % TPR) * SEG;
+    const int y = ty * ROWS + tid / TPR;
+    const int z0 = tz * zc, z1 = min(D, z0 + zc);
+    const int nval = min(SEG, W - xs);
+    if (y < H && nval > 0) {
+        const bool vec = nval == SEG && (W & 3) == 0;          // 16-byte aligned rows
+        for (int z = z0; z < z1; ++z) {
+            const size_t v0 = ((size_t)z * H + y) * W + xs;
+            int lv[SEG];
+            uint32_t cv[SEG];
+            if (vec) {
+                const int4* lp = reinterpret_cast<const int4*>(labels + v0);
+                const uint4* cp = reinterpret_cast<const uint4*>(lab + v0);
 #pragma unroll
-            for (int q = 0; q < kSeg / 4; ++q) {
-                int4 a = lp[q];
-                uint4 c = cp[q];
-                lv[4 * q] = a.x; lv[4 * q + 1] = a.y; lv[4 * q + 2] = a.z; lv[4 * q + 3] = a.w;
-                cv[4 * q] = c.x; cv[4 * q + 1] = c.y; cv[4 * q + 2] = c.z; cv[4 * q + 3] = c.w;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < kSeg; ++q) {
-                bool ok = v0 + q < V;
-                lv[q] = ok ? labels[v0 + q] : 0;
-                cv[q] = ok ? lab[v0 + q] : 0u;
-            }
-        }
-        const int nval = (int)min((int64_t)kSeg, V - v0);
-        int x = (int)(v0 % W);
-        int64_t r = v0 / W;
-        int y = (int)(r % H);
-        int z = (int)(r / H);
-        int cur = lv[0];
-        int xa = x;
-        unsigned sl = cv[0] & 255u, sa = (cv[0] >> 8) & 255u, sb = (cv[0] >> 16) & 255u;
-#pragma unroll
-        for (int q = 1; q < kSeg; ++q) {
-            if (q < nval) {
-                int nx = x + 1, ny = y, nz = z;
-                if (nx == W) { nx = 0; ++ny; if (ny == H) { ny = 0; ++nz; } }
-                int l = lv[q];
-                if (l != cur || nx == 0) {
-                    run_flush(cur, A, xa, x, y, z, sl, sa, sb, s_key, s_acc, sums);
-                    cur = l;
-                    xa = nx;
-                    sl = 0u; sa = 0u; sb = 0u;
+                for (int q = 0; q < SEG / 4; ++q) {
+                    const int4 a = lp[q];
+                    const uint4 c = cp[q];
+                    lv[4 * q] = a.x; lv[4 * q + 1] = a.y; lv[4 * q + 2] = a.z; lv[4 * q + 3] = a.w;
+                    cv[4 * q] = c.x; cv[4 * q + 1] = c.y; cv[4 * q + 2] = c.z; cv[4 * q + 3] = c.w;
                 }
-                uint32_t c = cv[q];
-                sl += c & 255u; sa += (c >> 8) & 255u; sb += (c >> 16) & 255u;
-                x = nx; y = ny; z = nz;
+            } else {
+#pragma unroll
+                for (int q = 0; q < SEG; ++q) {
+                    const bool ok = q < nval;
+                    lv[q] = ok ? labels[v0 + q] : 0;
+                    cv[q] = ok ? lab[v0 + q] : 0u;
+                }
             }
+            int cur = lv[0];
+            int xa = xs;
+            unsigned sl = cv[0] & 255u, sa = (cv[0] >> 8) & 255u, sb = (cv[0] >> 16) & 255u;
+#pragma unroll
+            for (int q = 1; q < SEG; ++q) {
+                if (q < nval) {
+                    const int l = lv[q];
+                    if (l != cur) {
+                        run_flush(cur, A, xa, xs + q - 1, y, z, sl, sa, sb, s_key, s_acc, sums);
+                        cur = l;
+                        xa = xs + q;
+                        sl = 0u; sa = 0u; sb = 0u;
+                    }
+                    const uint32_t c = cv[q];
+                    sl += c & 255u; sa += (c >> 8) & 255u; sb += (c >> 16) & 255u;
+                }
+            }
+            run_flush(cur, A, xa, xs + nval - 1, y, z, sl, sa, sb, s_key, s_acc, sums);
         }
-        run_flush(cur, A, xa, x, y, z, sl, sa, sb, s_key, s_acc, sums);
     }
     __syncthreads();
-    int key = s_key[tid];
+    const int key = s_key[tid];
     if (key != kEmpty) {
         unsigned long long* o = sums + (size_t)key * kSumWords;
 #pragma unroll
         for (int i = 0; i < 7; ++i) atomicAdd(o + i, s_acc[tid][i]);
-    }
-}
-
-
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// One voxel per lane over the linear volume.  Lanes of a wave that share a
-// slot are summed with cross-lane adds and the wave's first such lane issues
-// one 64-bit atomic per sum, so the totals are exact and order-free.
-__global__ __launch_bounds__(256) void k_reduce(int W, int H, int64_t V, int A, const int32_t* __restrict__ labels,
-                                                const uint32_t* __restrict__ lab, unsigned long long* __restrict__ sums)
-{
-    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = v < V;
-    int slot = -1;
-    int vx = 0, vy = 0, vz = 0, vL = 0, va = 0, vb = 0;
-    if (valid) {
-        int l = labels[v];
-        slot = l < 0 ? A : l;
-        int x = (int)(v % W);
-        int64_t r = v / W;
-        int y = (int)(r % H);
-        int z = (int)(r / H);
-        uint32_t pk = lab[v];
-        vx = 2 * x + 1; vy = 2 * y + 1; vz = 2 * z + 1;
-        vL = pk & 255u; va = (pk >> 8) & 255u; vb = (pk >> 16) & 255u;
-    }
-    int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(valid);
-    while (pending) {
-        int leader = __ffsll((long long)pending) - 1;
-        int s = __shfl(slot, leader, 64);
-        bool mine = valid && slot == s;
-        unsigned long long mask = __ballot(mine);
-        int cx = wave_sum(mine ? vx : 0);
-        int cy = wave_sum(mine ? vy : 0);
-        int cz = wave_sum(mine ? vz : 0);
-        int cL = wave_sum(mine ? vL : 0);
-        int ca = wave_sum(mine ? va : 0);
-        int cb = wave_sum(mine ? vb : 0);
-        if (lane == leader) {
-            unsigned long long* o = sums + (size_t)s * kSumWords;
-            atomicAdd(o + 0, (unsigned long long)cx);
-            atomicAdd(o + 1, (unsigned long long)cy);
-            atomicAdd(o + 2, (unsigned long long)cz);
-            atomicAdd(o + 3, (unsigned long long)__popcll(mask));
-            atomicAdd(o + 4, (unsigned long long)cL);
-            atomicAdd(o + 5, (unsigned long long)ca);
-            atomicAdd(o + 6, (unsigned long long)cb);
-        }
-        pending &= ~mask;
-        if (mine) valid = false;
     }
 }
 
@@ -1337,10 +1283,28 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums)
 {
-    int64_t V = (int64_t)d.W * d.H * d.D;
-    int64_t per_block = 256 * kSeg * kRedChunks;
-    hipLaunchKernelGGL(k_reduce_runs, dim3(cdiv(V, per_block)), dim3(256), 0, st, d.W, d.H, V, d.A, labels, lab,
-                       sums);
+    // the widest rows per thread (16, 8 or 4 voxels) that still give about
+    // four blocks per CU, then frames per brick to keep the block count
+    // near 2048 on long clips
+    const int tiles_x = (int)cdiv(d.W, kRedTileW);
+    int seg = 16;
+    while (seg > 4 && (int64_t)tiles_x * cdiv(d.H, 256 / (kRedTileW / seg)) * d.D < 1024) seg >>= 1;
+    const int rows = 256 / (kRedTileW / seg);
+    const int tiles_y = (int)cdiv(d.H, rows);
+    const int64_t per_frame = (int64_t)tiles_x * tiles_y;
+    int zc = (int)(per_frame * d.D / 2048);
+    zc = zc < 1 ? 1 : (zc > 8 ? 8 : zc);
+    const int64_t nb = per_frame * cdiv(d.D, zc);
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (seg == 16)
+        hipLaunchKernelGGL(k_reduce_bricks<16>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
+                           tiles_y, zc, labels, lab, sums);
+    else if (seg == 8)
+        hipLaunchKernelGGL(k_reduce_bricks<8>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
+                           tiles_y, zc, labels, lab, sums);
+    else
+        hipLaunchKernelGGL(k_reduce_bricks<4>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
+                           tiles_y, zc, labels, lab, sums);
     return hipGetLastError();
 }
 
