@@ -174,6 +174,7 @@ struct asv_ctx {
     int map_cap = 0;
     long long* post_scratch = nullptr;
     int64_t post_cap = 0;
+    unsigned long long* ck_word = nullptr;
 
     int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
     int depth_cap = 0;      // frames the volume buffers hold
@@ -186,7 +187,7 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace, c->map_table, c->post_scratch};
+                    c->trace, c->map_table, c->post_scratch, c->ck_word};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -644,7 +645,7 @@ int asv_download_labels(asv_ctx* c, int32_t* labels, void* stream)
 }
 
 int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table, int32_t n_frames,
-                               int32_t* labels, void* stream)
+                               int32_t* labels, int64_t index0, uint64_t* checksum, void* stream)
 {
     if (!c || !table || !labels || n_table < 1 || n_frames < 0 || n_frames > c->d.D)
         return fail(ASV_ERR_ARG, "bad label map arguments");
@@ -662,6 +663,17 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
     int32_t* scratch = c->labels[1 - c->cur];          // free after the run
     HIP_TRY(launch_map_labels(st, c->labels[c->cur], scratch, n, c->map_table, n_table));
     HIP_TRY(hipMemcpyAsync(labels, scratch, (size_t)n * 4, hipMemcpyDefault, st));
+    if (checksum) {
+        if (!c->ck_word) HIP_TRY(hipMalloc((void**)&c->ck_word, sizeof(unsigned long long)));
+        unsigned long long* ck = c->ck_word;
+        unsigned long long host = 0ull;
+        HIP_TRY(launch_zero_u64(st, reinterpret_cast<uint64_t*>(ck), 1));
+        HIP_TRY(launch_checksum(st, scratch, n, index0, ck));
+        HIP_TRY(hipMemcpyAsync(&host, ck, sizeof(host), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        *checksum = host;
+        return ASV_OK;
+    }
     HIP_TRY(hipStreamSynchronize(st));
     return ASV_OK;
 }

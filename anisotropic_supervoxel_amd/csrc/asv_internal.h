@@ -50,6 +50,9 @@ hipError_t launch_finalize(hipStream_t st, const Dims& d, const unsigned long lo
 hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, int64_t n, const int32_t* table,
                              int n_table);
 
+// *out += sum over i < n of splitmix64(((index0 + i) << 32) | (uint32)labels[i]).
+hipError_t launch_checksum(hipStream_t st, const int32_t* labels, int64_t n, int64_t index0, unsigned long long* out);
+
 // Device post-processing: mode 0 fills unlabelled voxels from the previous
 // labelled voxel after mapping through `table` (label_replace composition),
 // mode 1 is the dead-seed rank remap.  `scratch` holds post_scratch_words(V)

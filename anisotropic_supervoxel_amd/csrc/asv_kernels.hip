@@ -1337,6 +1337,35 @@ hipError_t launch_map_labels(hipStream_t st, const int32_t* in, int32_t* out, in
     return hipGetLastError();
 }
 
+// Order-free checksum of a label run: sum over voxels of
+// splitmix64((global index << 32) | (uint32)label), modulo 2^64, added into
+// *out.  Integer sums, so the value is exact and launch-shape independent.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_checksum(const int32_t* __restrict__ labels, int64_t n, int64_t index0,
+                                                  unsigned long long* __restrict__ out)
+{
+    unsigned long long acc = 0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += splitmix64(((unsigned long long)(index0 + i) << 32) | (unsigned)labels[i]);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+hipError_t launch_checksum(hipStream_t st, const int32_t* labels, int64_t n, int64_t index0, unsigned long long* out)
+{
+    unsigned blocks = n > 0 ? cdiv(n, 256) : 1;
+    if (blocks > 4096u) blocks = 4096u;
+    hipLaunchKernelGGL(k_checksum, dim3(blocks), dim3(256), 0, st, labels, n, index0, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // post-processing of the final labels on the device
 // (check_dead_seeds / label_replace.gs / sequence_label, GPU-AS/Image.cpp:1579-1778)

@@ -33,16 +33,6 @@ double ms_since(clock_t_::time_point t0)
     return std::chrono::duration<double, std::milli>(clock_t_::now() - t0).count();
 }
 
-uint64_t fnv1a(const int32_t* p, size_t n, uint64_t h)
-{
-    const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n * 4; ++i) {
-        h ^= b[i];
-        h *= 1099511628211ull;
-    }
-    return h;
-}
-
 asv_stream_params stream_params_of(const asv_serve_params* sp, const asv_serve_request* r)
 {
     asv_stream_params p;
@@ -114,7 +104,8 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     int64_t active_bytes = 0;
     // Idle resident contexts left by finished streams, oldest first.  They
     // count against the memory budget and are freed when admission needs
-    // the room or more than max_active of them wait.
+    // the room or more than 4 x max_active of them wait (hipFree waits for
+    // the whole device, so the loop frees as little as it can).
     struct Idle {
         asv_ctx* c;
         int64_t bytes;
@@ -133,7 +124,13 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
 
     // A new stream adopts an idle context of its frame size and settings
     // when one is there (no device allocation on admission).
-    auto process = [&](int ri, hipStream_t st) -> int {
+    // Each worker delivers a window's finished frames into its own
+    // page-locked buffer (one transfer), grown to the largest window seen.
+    struct WindowOut {
+        int32_t* p = nullptr;
+        size_t cap = 0;
+    };
+    auto process = [&](int ri, hipStream_t st, WindowOut& wo) -> int {
         const asv_serve_request& r = reqs[ri];
         asv_stream_params p = stream_params_of(sp, &r);
         asv_stream* s = nullptr;
@@ -152,7 +149,19 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         std::vector<int32_t> iv((size_t)nb * 2);
         asv_stream_plan(&p, iv.data(), nb);
         const size_t plane = (size_t)r.width * r.height;
-        std::vector<int32_t> labels(plane * (size_t)r.frames, -1);
+        size_t max_nf = 0;
+        for (int b = 0; b < nb; ++b) max_nf = std::max(max_nf, (size_t)(iv[2 * (size_t)b + 1] - iv[2 * (size_t)b]));
+        if (max_nf * plane > wo.cap) {
+            if (wo.p) (void)hipHostFree(wo.p);
+            wo.p = nullptr;
+            wo.cap = 0;
+            if (hipHostMalloc((void**)&wo.p, max_nf * plane * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+                wo.p = nullptr;
+                asv_stream_destroy(s);
+                return ASV_ERR_NOMEM;
+            }
+            wo.cap = max_nf * plane;
+        }
         uint64_t vox_iters = 0;
         double ready = ms_since(t0);
         for (int b = 0; b < nb && !rc; ++b) {
@@ -187,7 +196,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             w.ready_ms = ready;
             w.start_ms = ms_since(t0);
             int32_t done = 0;
-            rc = asv_stream_block(s, b, rgb, fwd, bwd, labels.data(), &done, st);
+            rc = asv_stream_block_into(s, b, rgb, fwd, bwd, wo.p, &done, st);
             w.end_ms = ms_since(t0);
             int32_t info[6];
             asv_stream_block_info(s, info);
@@ -197,7 +206,9 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             vox_iters += (uint64_t)plane * nf * (uint64_t)p.lloyd_iters;
         }
         if (!rc) {
-            results[ri].labels_hash = fnv1a(labels.data(), labels.size(), 1469598103934665603ull);
+            uint64_t ck = 0;
+            asv_stream_checksum(s, &ck);
+            results[ri].labels_hash = ck;
             results[ri].voxel_iters = (int64_t)vox_iters;
         }
         asv_ctx* left = rc ? nullptr : asv::stream_take_ctx(s);
@@ -208,7 +219,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
                 std::lock_guard<std::mutex> g(mu);
                 idle.push_back(Idle{left, asv_device_bytes(left)});
                 idle_bytes += idle.back().bytes;
-                if ((int)idle.size() > sp->max_active) {
+                if ((int)idle.size() > 4 * sp->max_active) {
                     evict = idle.front().c;
                     idle_bytes -= idle.front().bytes;
                     idle.pop_front();
@@ -223,6 +234,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         (void)hipSetDevice(dev);
         hipStream_t st = nullptr;
         (void)hipStreamCreate(&st);
+        WindowOut wo;
         for (;;) {
             int ri;
             {
@@ -232,7 +244,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
                 ri = work.front();
                 work.pop_front();
             }
-            int rc = process(ri, st);
+            int rc = process(ri, st, wo);
             {
                 std::lock_guard<std::mutex> g(mu);
                 results[ri].status = rc;
@@ -242,6 +254,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             }
             cv.notify_all();
         }
+        if (wo.p) (void)hipHostFree(wo.p);
         if (st) (void)hipStreamDestroy(st);
     };
 

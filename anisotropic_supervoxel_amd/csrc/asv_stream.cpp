@@ -91,6 +91,7 @@ struct asv_stream {
     // resident device window: one context reused across blocks
     asv_ctx* ctx = nullptr;
     int depth_cap = 0, seed_cap = 0, prev_depth = 0;
+    uint64_t checksum = 0;                // of the frames finished so far
     ~asv_stream()
     {
         if (ctx) asv_destroy(ctx);
@@ -215,7 +216,17 @@ int asv_stream_block_info(const asv_stream* s, int32_t* info)
 int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
                      int32_t* labels_all, int32_t* frames_done, void* stream)
 {
-    if (!s || !rgb || !fwd || !bwd || !labels_all) return sfail(ASV_ERR_ARG, "null argument");
+    if (!s || !labels_all) return sfail(ASV_ERR_ARG, "null argument");
+    if (block < 0 || 2 * (size_t)block >= s->intervals.size()) return sfail(ASV_ERR_STATE, "no such block");
+    const size_t start = (size_t)s->intervals[2 * (size_t)block];
+    return asv_stream_block_into(s, block, rgb, fwd, bwd, labels_all + start * s->p.width * s->p.height,
+                                 frames_done, stream);
+}
+
+int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
+                          int32_t* out, int32_t* frames_done, void* stream)
+{
+    if (!s || !rgb || !fwd || !bwd || !out) return sfail(ASV_ERR_ARG, "null argument");
     const int nblocks = (int)s->intervals.size() / 2;
     if (block != s->next_block || block >= nblocks) return sfail(ASV_ERR_STATE, "blocks must be processed in order");
     const int W = s->p.width, H = s->p.height, B = s->p.block_size, O = s->p.overlap, S = B - O;
@@ -282,14 +293,17 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
     if (rc) return rc;
     s->prev_depth = Db;
     const int done_frames = last ? Db : Db - O;
-    int32_t* dst = labels_all + (size_t)start * plane;
+    int32_t* dst = out;
 
     // check_dead_seeds (streamGPU-AS/Image.cpp:1534-1579) on the device, then
     // the block2all stitch (:559-563), where an id absent from the map reads
     // as 0 (std::map operator[]); only the stitched frames come back.
     int dead = 0, nfin = 0;
     rc = asv_postprocess_device(ctx, sums.data(), &dead, &nfin, stream);
-    if (!rc) rc = asv_download_labels_mapped(ctx, block2all.data(), A, done_frames, dst, stream);
+    uint64_t ck = 0;
+    if (!rc)
+        rc = asv_download_labels_mapped(ctx, block2all.data(), A, done_frames, dst, (int64_t)start * (int64_t)plane,
+                                        &ck, stream);
     if (rc) return rc;
     std::vector<KeptSeed> alive;
     for (int i = 0; i < A; ++i) {
@@ -316,6 +330,7 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
     }
     s->kept.swap(next);
     s->prev_map.swap(block2all);
+    s->checksum += ck;
     s->last_A = A;
     s->last_dead = dead;
     s->last_budget = budget;
@@ -342,6 +357,13 @@ int asv_stream_run_video(const asv_stream_params* p, const uint8_t* rgb, const f
     if (!rc && seeds_placed) *seeds_placed = s->placed;
     asv_stream_destroy(s);
     return rc;
+}
+
+int asv_stream_checksum(const asv_stream* s, uint64_t* out)
+{
+    if (!s || !out) return sfail(ASV_ERR_ARG, "null argument");
+    *out = s->checksum;
+    return ASV_OK;
 }
 
 const char* asv_stream_last_error(void) { return g_stream_err.c_str(); }

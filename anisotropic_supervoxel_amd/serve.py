@@ -142,10 +142,15 @@ def summarize(results, windows, wall_ms):
     }
 
 
-def fnv1a_labels(labels):
-    """FNV-1a 64 over int32 labels (the hash asv_serve_run reports)."""
-    h = 1469598103934665603
-    for b in np.ascontiguousarray(labels, np.int32).tobytes():
-        h ^= b
-        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
-    return h
+def labels_checksum(labels):
+    """The checksum asv_serve_run reports per stream (asv_stream_checksum):
+    sum over voxels i of splitmix64((i << 32) | uint32(label_i)) mod 2^64,
+    with i the voxel's index in the whole video."""
+    lab = np.ascontiguousarray(labels, np.int32).reshape(-1)
+    x = (np.arange(lab.size, dtype=np.uint64) << np.uint64(32)) | lab.view(np.uint32).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        x = x + np.uint64(0x9E3779B97F4A7C15)
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return int(x.sum(dtype=np.uint64))

@@ -12,12 +12,19 @@ Prints one JSON line.  `value` is Mvoxel-iters/s = V * iters * steps / time.
 bytes per voxel (int32 label in, packed LAB in, int32 label out) over the
 pass's average launch time from HIP events on the launch stream.
 `cpu_baseline` times the CPU oracle on a bounded sample of the same video.
+`served` is the streaming form: the plain per-window loop against the
+served loop with continuous batching over concurrent streams, aggregate
+Mvoxel-iters/s and p50/p99 window latency for each (see served()).
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+# Eight hardware queues, so that the served leg's eight worker streams do
+# not share queues (HIP's default is four).  Must precede HIP start-up.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -68,6 +75,41 @@ def cpu_baseline(seed, W, H, D, ell, vmax, sigma, K, iters, frames):
                       f"{dt:.1f} s wall"}
 
 
+def served(n_streams, actives, seed):
+    """Served streaming workload: n_streams requests with Poisson arrivals at
+    40/s (resolutions 320x240 / 640x480 / 1280x720, 60-300 frames, density
+    1-4 supervoxels per 1000 pixels per frame), windows of 10 frames with
+    stride 5, 5 Lloyd iterations per window, seeds carried across windows.
+    Frames are generated up front into page-locked host memory.  Runs the
+    plain per-window loop (one stream at a time, no overlap) and the served
+    loop (each of `actives` concurrent-stream limits, admission by
+    footprint), and checks that every stream's stitched-label checksum is
+    the same in every mode.  Both loops deliver each window's finished
+    labels to host memory and keep the checksum on the device."""
+    from anisotropic_supervoxel_amd import serve
+    reqs = serve.make_requests(n_streams, rate_per_s=40.0, seed=seed)
+    serve.attach_frames(reqs)
+    out = {}
+    hashes = {}
+    modes = [("plain", dict(max_active=1, honor_arrivals=0))] + [(f"served{a}", dict(max_active=a)) for a in actives]
+    for name, kw in modes:
+        res, w, wall = serve.run(serve.default_params(**kw), reqs)
+        s = serve.summarize(res, w, wall)
+        hashes[name] = [r["labels_hash"] for r in res]
+        out[name] = {"mvoxel_iters_per_s": round(s["mvoxel_iters_per_s"], 1), "p50_window_ms": round(s["p50_window_ms"], 2),
+                     "p99_window_ms": round(s["p99_window_ms"], 2), "wall_ms": round(wall, 1), "windows": s["windows"]}
+    plain = out["plain"]["mvoxel_iters_per_s"]
+    return {
+        "workload": f"{n_streams} streams, Poisson 40/s, 320x240/640x480/1280x720, 60-300 frames, density 1-4 per "
+                    "1000 px per frame, windows 10 stride 5, 5 Lloyd iterations",
+        "plain_per_window_loop": out["plain"],
+        "served": [dict(max_active=a, **out[f"served{a}"],
+                        served_over_plain=round(out[f"served{a}"]["mvoxel_iters_per_s"] / plain, 3)) for a in actives],
+        "labels_identical_alone_and_in_mix": all(hashes[f"served{a}"] == hashes["plain"] for a in actives),
+        "voxel_iters": int(sum(r["voxel_iters"] for r in res)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +118,9 @@ def main():
     ap.add_argument("--workload", default="plain-640x480x200-K10000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=70)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-streams", type=int, default=24)
+    ap.add_argument("--served-active", default="4,8", help="comma-separated concurrent-stream limits")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB torch copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -163,8 +208,11 @@ def main():
             "reduction_GBps": round(8.0 * V / (red_avg_ms * 1e-3) / 1e9, 1),
         },
         "cpu_baseline": None,
+        "served": None,
     }
     clip.close()
+    if not args.no_served:
+        out["served"] = served(args.served_streams, [int(a) for a in args.served_active.split(",")], seed=5)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(seed, W, H, D, ell, vmax, sigma, K, iters, args.cpu_frames)
     print(json.dumps(out), flush=True)

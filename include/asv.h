@@ -151,10 +151,13 @@ int asv_download_sums(asv_ctx* ctx, int64_t* sums, void* stream);
 /* The first n_frames frames of the final labels mapped through table
  * (label l -> table[l], labels outside [0, n_table) -> 0) on the device,
  * then copied out: the block2all stitch of the streaming program
- * (streamGPU-AS/Render.cpp:559-563) without a full label download.
- * Synchronises stream. */
+ * (streamGPU-AS/Render.cpp:559-563) without a full label download.  When
+ * checksum is not NULL it receives the sum, modulo 2^64, over the copied
+ * voxels i of splitmix64(((index0 + i) << 32) | (uint32_t)label), computed
+ * on the device (an order-free fingerprint; index0 is the first voxel's
+ * index in the whole video).  Synchronises stream. */
 int asv_download_labels_mapped(asv_ctx* ctx, const int32_t* table, int32_t n_table, int32_t n_frames,
-                               int32_t* labels, void* stream);
+                               int32_t* labels, int64_t index0, uint64_t* checksum, void* stream);
 
 /* check_dead_seeds (GPU-AS/Image.cpp:1579-1720) on the device: from the
  * final counts in sums ((A+1) x 8 as asv_download_sums writes them), the
@@ -260,6 +263,14 @@ int asv_stream_block_info(const asv_stream* s, int32_t* info6);
  * labels_all (W x H x total_frames, GL rows, global ids). */
 int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
                      int32_t* labels_all, int32_t* frames_done, void* stream);
+/* The same block, with its finished frames written from `out` onward
+ * (frames_done x W x H labels), for callers that deliver results window by
+ * window (page-locked `out` makes the copy one transfer). */
+int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
+                          int32_t* out, int32_t* frames_done, void* stream);
+/* Checksum of every frame finished so far (stitched global labels, voxel
+ * index over the whole video), as asv_download_labels_mapped defines it. */
+int asv_stream_checksum(const asv_stream* s, uint64_t* out);
 /* All blocks of a whole video (arrays cover total_frames frames). */
 int asv_stream_run_video(const asv_stream_params* p, const uint8_t* rgb, const float* fwd, const float* bwd,
                          int32_t* labels_all, int32_t* seeds_placed, void* stream);
@@ -304,7 +315,7 @@ typedef struct asv_serve_window {
 typedef struct asv_serve_result {
     int32_t stream_id;
     int32_t status;             /* ASV_OK, ASV_ERR_ADMIT (never fits the budget) or an error */
-    uint64_t labels_hash;       /* FNV-1a 64 of the stitched global labels (GL rows) */
+    uint64_t labels_hash;       /* asv_stream_checksum of the stitched global labels */
     int64_t footprint;          /* device bytes admitted for the stream */
     int64_t voxel_iters;
     double  admitted_ms, done_ms;

@@ -53,7 +53,7 @@ def test_served_mix_equals_alone_and_oracle(have_gpu):
                                            r["vmax"], r["noise_sigma"])
         seeds100 = int(r["density"] * r["width"] * r["height"] / 10.0)
         want, _ = ostream.run_video(rgb, fwd, bwd, 1, block_size=10, overlap=5, iters=5, seeds_per_100=seeds100)
-        assert serve.fnv1a_labels(want) == by_id[r["stream_id"]]["labels_hash"]
+        assert serve.labels_checksum(want) == by_id[r["stream_id"]]["labels_hash"]
     s = serve.summarize(mix, windows, wall)
     assert s["windows"] == len(windows) > 0 and s["p99_window_ms"] >= s["p50_window_ms"] > 0
 
@@ -89,3 +89,57 @@ def test_reused_context_equals_fresh(have_gpu):
     for i, r in enumerate(reqs):
         alone, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), [r])
         assert alone[0]["labels_hash"] == seq[i]["labels_hash"], r
+
+
+def _splitmix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def test_labels_checksum_restatement():
+    """The numpy checksum against scalar SplitMix64 (Steele, Lea and Flood,
+    2014; first output for seed 0 is 0xE220A8397B1DCDAF)."""
+    assert _splitmix64(0) == 0xE220A8397B1DCDAF
+    lab = np.array([[3, -1, 0], [7, 7, 2 ** 31 - 1]], np.int32)
+    want = sum(_splitmix64((i << 32) | (int(v) & 0xFFFFFFFF)) for i, v in enumerate(lab.reshape(-1))) % (1 << 64)
+    assert serve.labels_checksum(lab) == want
+
+
+@pytest.mark.gpu
+def test_stream_checksum_matches_stitched_labels(have_gpu):
+    """asv_stream_block_into window by window: the device checksum equals the
+    checksum of the stitched labels, which equal the oracle's."""
+    import ctypes
+    W, H, D, K = 48, 40, 37, 120
+    rgb, fwd, bwd, _ = asv.synth_video(321, W, H, D, 6, 2.0, 4.0)
+    sp = api.stream_params(W, H, D, K, block_size=10, overlap=5, lloyd_iters=4)
+    lib = asv._lib.load()
+    s = ctypes.c_void_p()
+    asv._lib.check(lib.asv_stream_create(ctypes.byref(sp), ctypes.byref(s)))
+    nb = lib.asv_stream_block_count(s)
+    iv = np.zeros(2 * nb, np.int32)
+    lib.asv_stream_plan(ctypes.byref(sp), iv.ctypes.data, nb)
+    out = np.full((D, H, W), -9, np.int32)
+    pos = 0
+    try:
+        for b in range(nb):
+            f0 = int(iv[2 * b])
+            win = np.zeros((int(iv[2 * b + 1]) - f0, H, W), np.int32)
+            done = ctypes.c_int32(0)
+            rc = lib.asv_stream_block_into(s, b, rgb[f0:].ctypes.data, fwd[f0:].ctypes.data, bwd[f0:].ctypes.data,
+                                           win.ctypes.data, ctypes.byref(done), None)
+            assert rc == 0, lib.asv_stream_last_error()
+            assert f0 == pos or b == 0
+            out[pos:pos + done.value] = win[:done.value]
+            pos += done.value
+        ck = ctypes.c_uint64(0)
+        asv._lib.check(lib.asv_stream_checksum(s, ctypes.byref(ck)))
+    finally:
+        lib.asv_stream_destroy(s)
+    assert pos == D
+    want, _ = ostream.run_video(rgb, fwd, bwd, K, 10, 5, 4)
+    assert np.array_equal(out, want)
+    assert ck.value == serve.labels_checksum(want)
