@@ -15,6 +15,20 @@ struct alignas(16) SeedRec {
     float4 a, b, c;
 };
 
+// One clip's share of a JFA pass launch (the marching kernel covers one or
+// several clips; work items item_begin .. of this clip are its 16 x 16
+// columns x K residues x nseg segments of seg slices).
+struct JfaClip {
+    const int32_t* in;
+    int32_t* out;
+    const uint32_t* lab;
+    const SeedRec* rec;
+    int W, H, D, A;
+    int tiles_x, tiles_y, seg, nseg;
+    int item_begin;
+    float ratio;
+};
+
 // Per-slot centroid sums: sum(2x+1), sum(2y+1), sum(2z+1), n, sum L, a, b, pad.
 constexpr int kSumWords = 8;
 

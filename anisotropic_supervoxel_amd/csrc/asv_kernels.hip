@@ -20,6 +20,17 @@
 
 namespace asv {
 
+// A pointer into global memory (address space 1): loads and stores through
+// it compile to global_* instructions rather than flat ones.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+// One 16-byte record field through a global pointer.
+__device__ __forceinline__ float4 ld_f4(gptr<const float> p)
+{
+    return make_float4(p[0], p[1], p[2], p[3]);
+}
+
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
@@ -765,37 +776,44 @@ __global__ __launch_bounds__(256) void k_jfa_tile(int W, int H, int D, int tiles
     out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
 }
 
-// Persistent tiled pass for the lowest-id rule when the seed count fits the
-// LDS id table (A <= kDirectIds).  Per 16 x 16 tile of one slice:
-//  1. each wave loads rows of the three slabs (z, z-K, z+K) over the tile
-//     grown by K, one lane per column, with clamped coordinates (so every
-//     later tap is a fixed offset and CLAMP_TO_EDGE is already applied);
-//  2. ids are claimed in a direct-mapped byte table slot[id] (read first,
-//     32-bit CAS only on an unclaimed byte); the claimer appends the id to
-//     the tile's candidate list;
-//  3. after a barrier the staged ids become 6-bit compact indices and the
-//     candidates' records are copied to LDS;
-//  4. each lane ORs its 27 taps into a 64-bit mask and evaluates each set
-//     bit once (exact de-duplication);
-//  5. the claimed bytes are reset for the next tile.
-// More than 63 candidates in a tile: per-tap path over global labels.
-// Blocks are persistent; XCD x (blocks b with b % 8 == x) sweeps one
-// contiguous z-major range of tiles.
+// LDS id table of the marching pass: one byte per seed id (A <= kDirectIds);
+// at most kListCap ids are listed per table generation.
 constexpr int kDirectIds = 16384;
 constexpr int kListCap = 512;
 constexpr unsigned kSlotEmpty = 0xFFu;
 constexpr unsigned kSlotClaimed = 0xFEu;
 
+// Marching persistent pass for the lowest-id rule (A <= kDirectIds,
+// K <= 16), also the batched form: one launch can cover several clips.
+//
+// A work item is one 16 x 16 column (tx, ty) of one clip, one residue
+// r = z mod K and a segment of up to `seg` slices z = z0, z0 + K, ...
+// Moving from z to z + K keeps two of the three slabs a voxel's taps read
+// (z and z + K become the new z - K and z), so each step stages, claims
+// and converts only the new slab clamp(z + 2K):
+//  - the direct-mapped byte table slot[id] and the candidate list persist
+//    along the march (an id claimed earlier keeps its compact index); when
+//    a step would take the list past 63 candidates, the table is released
+//    and rebuilt from the three current slabs;
+//  - every thread keeps, in registers, the 9-tap candidate masks of its
+//    voxel position in the three slabs, and computes only the new slab's
+//    (9 LDS bytes); the voxel's 27-tap mask is the OR of the three;
+//  - two barriers per step (claims done; new slab and records staged).
+// A rebuild that still leaves more than 63 candidates evaluates that slice
+// per tap from global labels.
+// Work items are ordered (segment, residue, ty, tx), tx fastest, and split
+// into 8 contiguous ranges, one per XCD (block b runs on XCD b mod 8), so
+// concurrently marching columns of one XCD are neighbours that share halo
+// rows in its L2.
 template <int K>
-__global__ __launch_bounds__(256) void k_jfa_direct(int W, int H, int D, int A, int tiles_x, int tiles_y, int ntiles,
-                                                    const int32_t* __restrict__ in, int32_t* __restrict__ out,
-                                                    const uint32_t* __restrict__ lab,
-                                                    const SeedRec* __restrict__ rec, float ratio)
+__global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
+                                                   int nitems)
 {
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
     constexpr int RA = R * R;
     constexpr int RPI = 64 / R;               // staged rows per wave instruction
     constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
+    constexpr unsigned long long kBadBit = 1ull << kMaxCand;
     __shared__ uint32_t s_slot[kDirectIds / 4];
     __shared__ uint8_t s_idx[3 * RA];
     __shared__ float4 s_rec[kMaxCand * 3];
@@ -805,170 +823,228 @@ __global__ __launch_bounds__(256) void k_jfa_direct(int W, int H, int D, int A, 
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int nwords = (A + 3) >> 2;
-    for (int i = tid; i < nwords; i += 256) s_slot[i] = 0xFFFFFFFFu;
+    for (int i = tid; i < kDirectIds / 4; i += 256) s_slot[i] = 0xFFFFFFFFu;
     if (tid == 0) s_count = 0;
     __syncthreads();
 
-    // persistent XCD-aware tile walk
     const int nb = gridDim.x;                 // a multiple of 8
     const int xcd = blockIdx.x & 7, slot_in_xcd = blockIdx.x >> 3, per_xcd = nb >> 3;
-    const int q = ntiles >> 3, rm = ntiles & 7;
-    const int t_beg = xcd * q + min(xcd, rm);
-    const int t_end = t_beg + q + (xcd < rm ? 1 : 0);
+    const int q = nitems >> 3, rm = nitems & 7;
+    const int i_beg = xcd * q + min(xcd, rm);
+    const int i_end = i_beg + q + (xcd < rm ? 1 : 0);
 
     const int sub = lane / R, col = lane - (lane / R) * R;
     const bool lane_ok = sub < RPI;
-    int ids[3 * NRS];
-    auto stage = [&](int t) {
-        const int sx0 = (t % tiles_x) * kTile, sy0 = ((t / tiles_x) % tiles_y) * kTile;
-        const int sz = t / (tiles_x * tiles_y);
-        const int gx = clampi(sx0 - K + col, 0, W - 1);
-#pragma unroll
-        for (int sl = 0; sl < 3; ++sl) {
-            const int gz = sl == 0 ? sz : clampi(sl == 1 ? sz - K : sz + K, 0, D - 1);
-            const int32_t* slab = in + (size_t)gz * H * W;
+    const int lx = (wave & 1) * 8 + (lane & 7);
+    const int ly = (wave >> 1) * 8 + (lane >> 3);
+    const int mbase = (ly + K) * R + (lx + K);
+
+    int known = 0;                            // s_count as of the last barrier
+
+    for (int item = i_beg + slot_in_xcd; item < i_end; item += per_xcd) {
+        JfaClip c = one;
+        if (clips) {
+            int lo = 0, hi = nclips - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (clips[mid].item_begin <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            c = clips[lo];
+        }
+        const int W = c.W, H = c.H, D = c.D;
+        // the descriptor's pointers are global memory (flat accesses would
+        // share the LDS wait counter)
+        const gptr<const int32_t> g_in = (gptr<const int32_t>)c.in;
+        const gptr<int32_t> g_out = (gptr<int32_t>)c.out;
+        const gptr<const uint32_t> g_lab = (gptr<const uint32_t>)c.lab;
+        int local = item - c.item_begin;
+        const int tx = local % c.tiles_x;
+        local /= c.tiles_x;
+        const int ty = local % c.tiles_y;
+        local /= c.tiles_y;
+        const int r = local % K;
+        const int g = local / K;
+        const int x0 = tx * kTile, y0 = ty * kTile;
+        const int z_first = r + K * g * c.seg;
+        if (z_first >= D) continue;
+        const int gx = clampi(x0 - K + col, 0, W - 1);
+        const int x = x0 + lx, y = y0 + ly;
+        const bool mine = x < W && y < H;
+
+        auto load_slab = [&](int gz, int* ids) {
+            const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
-                const int r = (wave + 4 * i) * RPI + sub;
+                const int rr = (wave + 4 * i) * RPI + sub;
                 int id = -1;
-                if (lane_ok && r < R) {
-                    const int gy = clampi(sy0 - K + r, 0, H - 1);
-                    id = slab[gy * W + gx];
-                }
-                ids[sl * NRS + i] = id;
+                if (lane_ok && rr < R) id = slab[clampi(y0 - K + rr, 0, H - 1) * W + gx];
+                ids[i] = id;
             }
-        }
-    };
-    if (t_beg + slot_in_xcd < t_end) stage(t_beg + slot_in_xcd);
-
-    for (int tile = t_beg + slot_in_xcd; tile < t_end; tile += per_xcd) {
-        const int tx = tile % tiles_x;
-        const int ty = (tile / tiles_x) % tiles_y;
-        const int z = tile / (tiles_x * tiles_y);
-        const int x0 = tx * kTile, y0 = ty * kTile;
-
-        // 1 + 2: claim the ids staged in registers (RPI rows per instruction,
-        // one lane per column); only the first lane of a run of equal ids
-        // (DPP row shift) tries, which keeps the CAS uncontended.  The rows
-        // of the next tile are loaded while this tile is evaluated.
+        };
+        auto claim = [&](const int* ids) {
 #pragma unroll
-        for (int n = 0; n < 3 * NRS; ++n) {
-            const int id = ids[n];
-            const int left = __builtin_amdgcn_update_dpp(-2, id, 0x111, 0xF, 0xF, false);
-            if (id >= 0 && id != left) {
-                const unsigned sh = (unsigned)(id & 3) * 8u;
-                uint32_t old = s_slot[id >> 2];
-                while (((old >> sh) & 0xFFu) == kSlotEmpty) {
-                    const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
-                    const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
-                    if (prev == old) {
-                        const int j = atomicAdd(&s_count, 1);
-                        if (j < kListCap) s_list[j] = id;
-                        const unsigned byte = j < kMaxCand ? (unsigned)j : (unsigned)(kMaxCand - 1);
-                        atomicAnd(&s_slot[id >> 2], ~(0xFFu << sh));
-                        atomicOr(&s_slot[id >> 2], byte << sh);
-                        break;
-                    }
-                    old = prev;
-                }
-            }
-        }
-        __syncthreads();
-        const int ncand = s_count;
-        const bool overflow = ncand > kMaxCand;
-
-        // 3: compact indices of the staged ids, candidate records
-        if (!overflow) {
-#pragma unroll
-            for (int sl = 0; sl < 3; ++sl)
-#pragma unroll
-                for (int i = 0; i < NRS; ++i) {
-                    const int r = (wave + 4 * i) * RPI + sub;
-                    const int id = ids[sl * NRS + i];
-                    if (lane_ok && r < R) {
-                        unsigned ci = (unsigned)kMaxCand;
-                        if (id >= 0) ci = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
-                        s_idx[sl * RA + r * R + col] = (uint8_t)ci;
+            for (int i = 0; i < NRS; ++i) {
+                const int id = ids[i];
+                const int left = __builtin_amdgcn_update_dpp(-2, id, 0x111, 0xF, 0xF, false);
+                if (id >= 0 && id != left) {
+                    const unsigned sh = (unsigned)(id & 3) * 8u;
+                    uint32_t old = s_slot[id >> 2];
+                    while (((old >> sh) & 0xFFu) == kSlotEmpty) {
+                        const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
+                        const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
+                        if (prev == old) {
+                            const int j = atomicAdd(&s_count, 1);
+                            if (j < kListCap) s_list[j] = id;
+                            const unsigned byte = j < kMaxCand ? (unsigned)j : (unsigned)(kMaxCand - 1);
+                            atomicAnd(&s_slot[id >> 2], ~(0xFFu << sh));
+                            atomicOr(&s_slot[id >> 2], byte << sh);
+                            break;
+                        }
+                        old = prev;
                     }
                 }
-            if (tid < ncand) {
-                const int id = s_list[tid];
-                SeedRec rr = rec[id];
-                s_rec[tid * 3 + 0] = rr.a;
-                s_rec[tid * 3 + 1] = rr.b;
-                s_rec[tid * 3 + 2] = rr.c;
-                s_rid[tid] = id;
             }
-        }
-        // prefetch the next tile's rows (ids is free once converted)
-        if (tile + per_xcd < t_end) stage(tile + per_xcd);
-        __syncthreads();
-
-        // 4: evaluate
-        const int lx = (wave & 1) * 8 + (lane & 7);
-        const int ly = (wave >> 1) * 8 + (lane >> 3);
-        const int x = x0 + lx, y = y0 + ly;
-        if (x < W && y < H) {
-            const size_t v = ((size_t)z * H + y) * W + x;
-            const uint32_t pk = lab[v];
-            const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
-            const float fx = (float)x, fy = (float)y, fz = (float)z;
-            unsigned long long best = ASV_KEY_INIT;
-            if (!overflow) {
-                const int base = (ly + K) * R + (lx + K);
-                unsigned long long mask = 0ull;
+        };
+        auto convert = [&](const int* ids, int slot) {
 #pragma unroll
-                for (int sl = 0; sl < 3; ++sl)
+            for (int i = 0; i < NRS; ++i) {
+                const int rr = (wave + 4 * i) * RPI + sub;
+                const int id = ids[i];
+                if (lane_ok && rr < R) {
+                    unsigned ci = (unsigned)kMaxCand;
+                    if (id >= 0) ci = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
+                    s_idx[slot * RA + rr * R + col] = (uint8_t)ci;
+                }
+            }
+        };
+        auto copy_records = [&](int from, int to) {
+            for (int j = from + tid; j < to; j += 256) {
+                const int id = s_list[j];
+                const gptr<const float> q = (gptr<const float>)(c.rec + id);
+                s_rec[j * 3 + 0] = ld_f4(q);
+                s_rec[j * 3 + 1] = ld_f4(q + 4);
+                s_rec[j * 3 + 2] = ld_f4(q + 8);
+                s_rid[j] = id;
+            }
+        };
+        auto slab_mask = [&](int slot) {
+            unsigned long long m = 0ull;
+            const uint8_t* p = s_idx + slot * RA + mbase;
 #pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy)
+            for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-                        for (int dx = -1; dx <= 1; ++dx)
-                            mask |= 1ull << s_idx[sl * RA + base + dy * K * R + dx * K];
-                mask &= ~(1ull << kMaxCand);
-                while (mask) {
-                    const int j = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1ull;
-                    const float d = cand_dist(s_rec[j * 3 + 0], s_rec[j * 3 + 1], s_rec[j * 3 + 2], fx, fy, fz, l0,
-                                              l1, l2, ratio);
-                    const unsigned long long kk = dkey(d, s_rid[j]);
-                    best = kk < best ? kk : best;
+                for (int dx = -1; dx <= 1; ++dx) m |= 1ull << p[dy * K * R + dx * K];
+            return m;
+        };
+        // release every claimed byte and empty the list (callers barrier after)
+        auto release = [&]() {
+            if (known <= kListCap) {
+                for (int j = tid; j < known; j += 256) {
+                    const int id = s_list[j];
+                    atomicOr(&s_slot[id >> 2], 0xFFu << ((unsigned)(id & 3) * 8u));
                 }
             } else {
-                const int xs[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
-                const int ys[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
-                const int zs[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
-                int seen0 = -1, seen1 = -1;
-                for (int sl = 0; sl < 3; ++sl)
-                    for (int r = 0; r < 3; ++r) {
-                        const int32_t* row = in + ((size_t)zs[sl] * H + ys[r]) * W;
-                        for (int c = 0; c < 3; ++c) {
-                            const int id = row[xs[c]];
-                            if (id < 0 || id == seen0 || id == seen1) continue;
-                            seen1 = seen0;
-                            seen0 = id;
-                            const SeedRec s = rec[id];
-                            const float d = cand_dist(s.a, s.b, s.c, fx, fy, fz, l0, l1, l2, ratio);
-                            const unsigned long long kk = dkey(d, id);
-                            best = kk < best ? kk : best;
-                        }
-                    }
+                for (int i = tid; i < kDirectIds / 4; i += 256) s_slot[i] = 0xFFFFFFFFu;
             }
-            out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
-        }
-        __syncthreads();
+        };
 
-        // 5: release the claimed bytes
-        if (ncand <= kListCap) {
-            for (int j = tid; j < ncand; j += 256) {
-                const int id = s_list[j];
-                atomicOr(&s_slot[id >> 2], 0xFFu << ((unsigned)(id & 3) * 8u));
+        unsigned long long m_lo = 0ull, m_mid = 0ull, m_hi = 0ull;   // slabs z-K, z, z+K
+        int ids_new[NRS];
+        bool full = true;
+        int z = z_first;
+        for (int step = 0; step < c.seg && z < D;) {
+            bool overflow = false;
+            if (full) {
+                release();
+                __syncthreads();
+                if (tid == 0) s_count = 0;
+                int ids0[NRS], ids1[NRS], ids2[NRS];
+                load_slab(clampi(z - K, 0, D - 1), ids0);
+                load_slab(z, ids1);
+                load_slab(clampi(z + K, 0, D - 1), ids2);
+                __syncthreads();
+                claim(ids0);
+                claim(ids1);
+                claim(ids2);
+                __syncthreads();
+                known = s_count;
+                overflow = known > kMaxCand;
+                if (!overflow) {
+                    convert(ids0, 0);
+                    convert(ids1, 1);
+                    convert(ids2, 2);
+                    copy_records(0, known);
+                }
+                __syncthreads();
+                if (!overflow) {
+                    m_lo = slab_mask(0);
+                    m_mid = slab_mask(1);
+                    m_hi = slab_mask(2);
+                }
+                full = overflow;              // rebuild again at the next slice
+            } else {
+                claim(ids_new);
+                __syncthreads();
+                const int cnt = s_count;
+                if (cnt > kMaxCand) {         // uniform: rebuild this slice
+                    known = cnt;
+                    full = true;
+                    continue;
+                }
+                convert(ids_new, 0);
+                copy_records(known, cnt);
+                known = cnt;
+                __syncthreads();
+                m_lo = m_mid;
+                m_mid = m_hi;
+                m_hi = slab_mask(0);
             }
-        } else {
-            for (int i = tid; i < nwords; i += 256) s_slot[i] = 0xFFFFFFFFu;
+            // the next step's new slab, loaded while this slice is evaluated
+            if (step + 1 < c.seg && z + K < D) load_slab(clampi(z + 2 * K, 0, D - 1), ids_new);
+
+            if (mine) {
+                const size_t v = ((size_t)z * H + y) * W + x;
+                const uint32_t pk = g_lab[v];
+                const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+                const float fx = (float)x, fy = (float)y, fz = (float)z;
+                unsigned long long best = ASV_KEY_INIT;
+                if (!overflow) {
+                    unsigned long long mask = (m_lo | m_mid | m_hi) & ~kBadBit;
+                    while (mask) {
+                        const int j = __ffsll((long long)mask) - 1;
+                        mask &= mask - 1ull;
+                        const float d = cand_dist(s_rec[j * 3 + 0], s_rec[j * 3 + 1], s_rec[j * 3 + 2], fx, fy, fz,
+                                                  l0, l1, l2, c.ratio);
+                        const unsigned long long kk = dkey(d, s_rid[j]);
+                        best = kk < best ? kk : best;
+                    }
+                } else {
+                    const int xs[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
+                    const int ys[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
+                    const int zs[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
+                    int seen0 = -1, seen1 = -1;
+                    for (int sl = 0; sl < 3; ++sl)
+                        for (int rr = 0; rr < 3; ++rr) {
+                            const gptr<const int32_t> row = g_in + ((size_t)zs[sl] * H + ys[rr]) * W;
+                            for (int cc = 0; cc < 3; ++cc) {
+                                const int id = row[xs[cc]];
+                                if (id < 0 || id == seen0 || id == seen1) continue;
+                                seen1 = seen0;
+                                seen0 = id;
+                                const gptr<const float> q = (gptr<const float>)(c.rec + id);
+                                const float d = cand_dist(ld_f4(q), ld_f4(q + 4), ld_f4(q + 8), fx, fy, fz, l0, l1,
+                                                          l2, c.ratio);
+                                const unsigned long long kk = dkey(d, id);
+                                best = kk < best ? kk : best;
+                            }
+                        }
+                }
+                g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
+            }
+            z += K;
+            ++step;
         }
-        if (tid == 0) s_count = 0;
-        __syncthreads();
     }
 }
 
@@ -1217,6 +1293,49 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
     return hipGetLastError();
 }
 
+// Resident blocks of k_jfa_march<K> (CUs x blocks per CU, a multiple of 8).
+template <int K>
+static int march_grid()
+{
+    static int grid_for_k[17] = {0};
+    if (grid_for_k[K] == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jfa_march<K>, 256, 0);
+        int g = cus * (per_cu > 0 ? per_cu : 1);
+        g = (g / 8) * 8;
+        grid_for_k[K] = g < 8 ? 8 : g;
+    }
+    return grid_for_k[K];
+}
+
+// Segment lengths for n clips of one launch: about four work items per
+// resident block overall, each clip's columns x residues cut into equal
+// segments of its ceil(D / K) slices.  Fills seg, nseg and item_begin;
+// returns the grid size.
+template <int K>
+static int march_shape(JfaClip* c, int n)
+{
+    const int grid = march_grid<K>();
+    int64_t columns = 0;
+    for (int i = 0; i < n; ++i) columns += (int64_t)c[i].tiles_x * c[i].tiles_y * K;
+    const int64_t target = 4 * (int64_t)grid;
+    const int64_t want = columns >= target ? 1 : (target + columns - 1) / columns;
+    int64_t items = 0;
+    for (int i = 0; i < n; ++i) {
+        const int steps = (c[i].D + K - 1) / K;
+        int nseg = (int)(want < steps ? want : steps);
+        if (nseg < 1) nseg = 1;
+        c[i].seg = (steps + nseg - 1) / nseg;
+        c[i].nseg = (steps + c[i].seg - 1) / c[i].seg;
+        c[i].item_begin = (int)items;
+        items += (int64_t)c[i].tiles_x * c[i].tiles_y * K * c[i].nseg;
+    }
+    const int64_t need = ((items + 7) / 8) * 8;
+    return (int)(grid < need ? grid : need);
+}
+
 template <int K>
 static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
                         const SeedRec* rec, float ratio)
@@ -1224,22 +1343,11 @@ static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     int nt = tx * ty * d.D;
     if (d.A <= kDirectIds) {
-        // persistent grid: resident blocks per CU x CUs, a multiple of 8
-        static int grid_for_k[17] = {0};
-        if (grid_for_k[K] == 0) {
-            int dev = 0, cus = 0, per_cu = 0;
-            (void)hipGetDevice(&dev);
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jfa_direct<K>, 256, 0);
-            int g = cus * (per_cu > 0 ? per_cu : 1);
-            g = (g / 8) * 8;
-            grid_for_k[K] = g < 8 ? 8 : g;
-        }
-        int g = grid_for_k[K];
-        int need = ((nt + 7) / 8) * 8;
-        if (g > need) g = need;
-        hipLaunchKernelGGL(k_jfa_direct<K>, dim3((unsigned)g), dim3(256), 0, st, d.W, d.H, d.D, d.A, tx, ty, nt, in,
-                           out, lab, rec, ratio);
+        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio};
+        const int g = march_shape<K>(&c, 1);
+        const int nitems = c.tiles_x * c.tiles_y * K * c.nseg;
+        hipLaunchKernelGGL(k_jfa_march<K>, dim3((unsigned)g), dim3(256), 0, st, c, (const JfaClip*)nullptr, 1,
+                           nitems);
         return;
     }
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
