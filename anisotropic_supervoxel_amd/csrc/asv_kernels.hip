@@ -814,6 +814,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     constexpr int RPI = 64 / R;               // staged rows per wave instruction
     constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
     constexpr unsigned long long kBadBit = 1ull << kMaxCand;
+    constexpr bool kLabAhead = K >= 3;        // LAB of the next slice loaded during this one
     __shared__ uint32_t s_slot[kDirectIds / 4];
     __shared__ uint8_t s_idx[3 * RA];
     __shared__ float4 s_rec[kMaxCand * 3];
@@ -951,6 +952,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 
         unsigned long long m_lo = 0ull, m_mid = 0ull, m_hi = 0ull;   // slabs z-K, z, z+K
         int ids_new[NRS];
+        uint32_t pk_next = 0u;
+        if (kLabAhead && mine) pk_next = g_lab[((size_t)z_first * H + y) * W + x];
         bool full = true;
         int z = z_first;
         for (int step = 0; step < c.seg && z < D;) {
@@ -1000,12 +1003,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 m_mid = m_hi;
                 m_hi = slab_mask(0);
             }
-            // the next step's new slab, loaded while this slice is evaluated
-            if (step + 1 < c.seg && z + K < D) load_slab(clampi(z + 2 * K, 0, D - 1), ids_new);
+            // the next step's new slab (and LAB), loaded while this slice is evaluated
+            const uint32_t pk_here = pk_next;
+            if (step + 1 < c.seg && z + K < D) {
+                load_slab(clampi(z + 2 * K, 0, D - 1), ids_new);
+                if (kLabAhead && mine) pk_next = g_lab[((size_t)(z + K) * H + y) * W + x];
+            }
 
             if (mine) {
                 const size_t v = ((size_t)z * H + y) * W + x;
-                const uint32_t pk = g_lab[v];
+                const uint32_t pk = kLabAhead ? pk_here : g_lab[v];
                 const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
                 const float fx = (float)x, fy = (float)y, fz = (float)z;
                 unsigned long long best = ASV_KEY_INIT;
