@@ -82,6 +82,15 @@ SIGNATURES = {
     "asv_serve_run": (ctypes.c_int, [_vp, _vp, _i32, _vp, _vp, _i32, _vp, _vp]),
     "asv_synth_video_range": (ctypes.c_int, [ctypes.c_uint64, _i32, _i32, _i32, _i32, _f32, _f32, _i32, _i32,
                                              _vp, _vp, _vp, _vp]),
+    "asv_batch_create": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
+    "asv_batch_destroy": (ctypes.c_int, [_vp]),
+    "asv_batch_size": (ctypes.c_int, [_vp]),
+    "asv_batch_clip": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_vp)]),
+    "asv_batch_run": (ctypes.c_int, [_vp, _vp]),
+    "asv_batch_set_autotune": (ctypes.c_int, [_vp, _i32]),
+    "asv_batch_launch_count": (ctypes.c_int, [_vp, _vp]),
+    "asv_batch_tuning": (ctypes.c_int, [_vp, _vp, _i32, _vp]),
+    "asv_batch_last_error": (ctypes.c_char_p, []),
     "asv_last_error": (ctypes.c_char_p, []),
     "asv_device_count": (ctypes.c_int, []),
 }

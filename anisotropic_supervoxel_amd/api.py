@@ -174,8 +174,26 @@ class Clip:
         self.s = self.lib.asv_grid_step(h)
         self.shape = (params.depth, params.height, params.width)
 
+    @classmethod
+    def borrowed(cls, handle, params, owner):
+        """A view of a context owned by someone else (a batch's clip); close()
+        leaves the context alone."""
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        self.p = params
+        self.stream = None
+        self.h = handle
+        self._owner = owner
+        self.A = self.lib.asv_seeds_placed(handle)
+        self.s = self.lib.asv_grid_step(handle)
+        self.shape = (params.depth, params.height, params.width)
+        return self
+
     # -- lifetime -----------------------------------------------------------
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             self.lib.asv_destroy(self.h)
             self.h = None

@@ -4,6 +4,7 @@
 #include "../../include/asv.h"
 #include "asv_internal.h"
 #include "asv_host_internal.h"
+#include "asv_ctx.h"
 
 #include <cmath>
 #include <cstdio>
@@ -140,46 +141,6 @@ bool valid_params(const asv_params* p)
 
 }  // namespace
 
-struct asv_ctx {
-    asv_params p{};
-    Dims d{};
-    int s = 0;
-    float ratio = 0.0f;
-    std::vector<int> steps;
-    std::vector<float> mult;
-    std::vector<int> grid_host;
-    std::vector<int4> grid_staging;
-    int64_t bytes = 0;
-    bool have_frames = false, have_flow = false, prepared = false;
-    bool trace_on = false;
-    int cur = 0;            // index of the label buffer holding the latest labels
-    int last_iter = 0;
-
-    uint8_t* rgb = nullptr;
-    float2* fwd = nullptr;
-    float2* bwd = nullptr;
-    float* lut = nullptr;
-    uint32_t* lab = nullptr;
-    float4* field = nullptr;
-    int* fix_list = nullptr;
-    int* fix_count = nullptr;
-    int32_t* labels[2] = {nullptr, nullptr};
-    int4* grid = nullptr;
-    int4* pos = nullptr;
-    float4* slab = nullptr;
-    SeedRec* rec = nullptr;
-    unsigned long long* sums[2] = {nullptr, nullptr};
-    float* trace = nullptr;
-    int32_t* map_table = nullptr;
-    int map_cap = 0;
-    long long* post_scratch = nullptr;
-    int64_t post_cap = 0;
-    unsigned long long* ck_word = nullptr;
-
-    int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
-    int depth_cap = 0;      // frames the volume buffers hold
-    int64_t V() const { return (int64_t)d.W * d.H * d.D; }
-};
 
 namespace {
 
@@ -520,8 +481,10 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
     if (iter == 1) {
         HIP_TRY(launch_seed_init(st, c->d, c->grid, c->lab, c->pos, c->slab));
         HIP_TRY(launch_zero_u64(st, reinterpret_cast<uint64_t*>(c->sums[1]), (int64_t)(c->d.A + 1) * kSumWords));
-        c->cur = 0;
     }
+    // the fill rewrites the whole volume, so every iteration can start in
+    // buffer 0 (pass p then reads buffer p & 1 in every iteration)
+    c->cur = 0;
     float* tr = c->trace_on && c->trace ? c->trace + (size_t)(iter - 1) * c->d.A * 12 : nullptr;
     int gate = (iter >= 2 && c->p.angle_gate) ? 1 : 0;
     HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->field, c->fwd, gate, c->rec, tr));

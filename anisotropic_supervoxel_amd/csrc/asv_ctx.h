@@ -1,0 +1,49 @@
+// asv_ctx.h -- the clip context behind the C ABI's opaque asv_ctx, shared
+// by the library's own translation units (asv_capi.cpp, asv_batch.cpp).
+#pragma once
+
+#include "../../include/asv.h"
+#include "asv_internal.h"
+
+#include <vector>
+
+struct asv_ctx {
+    asv_params p{};
+    asv::Dims d{};
+    int s = 0;
+    float ratio = 0.0f;
+    std::vector<int> steps;
+    std::vector<float> mult;
+    std::vector<int> grid_host;
+    std::vector<int4> grid_staging;
+    int64_t bytes = 0;
+    bool have_frames = false, have_flow = false, prepared = false;
+    bool trace_on = false;
+    int cur = 0;            // index of the label buffer holding the latest labels
+    int last_iter = 0;
+
+    uint8_t* rgb = nullptr;
+    float2* fwd = nullptr;
+    float2* bwd = nullptr;
+    float* lut = nullptr;
+    uint32_t* lab = nullptr;
+    float4* field = nullptr;
+    int* fix_list = nullptr;
+    int* fix_count = nullptr;
+    int32_t* labels[2] = {nullptr, nullptr};
+    int4* grid = nullptr;
+    int4* pos = nullptr;
+    float4* slab = nullptr;
+    asv::SeedRec* rec = nullptr;
+    unsigned long long* sums[2] = {nullptr, nullptr};
+    float* trace = nullptr;
+    int32_t* map_table = nullptr;
+    int map_cap = 0;
+    long long* post_scratch = nullptr;
+    int64_t post_cap = 0;
+    unsigned long long* ck_word = nullptr;
+
+    int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
+    int depth_cap = 0;      // frames the volume buffers hold
+    int64_t V() const { return (int64_t)d.W * d.H * d.D; }
+};

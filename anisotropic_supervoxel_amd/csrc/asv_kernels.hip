@@ -1317,17 +1317,18 @@ static int march_grid()
     return grid_for_k[K];
 }
 
-// Segment lengths for n clips of one launch: about four work items per
-// resident block overall, each clip's columns x residues cut into equal
-// segments of its ceil(D / K) slices.  Fills seg, nseg and item_begin;
-// returns the grid size.
+// Segment lengths for n clips of one launch: about per_block work items
+// per resident block overall, each clip's columns x residues cut into equal
+// segments of its ceil(D / K) slices.  grid_limit (> 0) caps the resident
+// blocks.  Fills seg, nseg and item_begin; returns the grid size.
 template <int K>
-static int march_shape(JfaClip* c, int n)
+static int march_shape(JfaClip* c, int n, int per_block = 4, int grid_limit = 0)
 {
-    const int grid = march_grid<K>();
+    int grid = march_grid<K>();
+    if (grid_limit > 0 && grid_limit < grid) grid = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
     int64_t columns = 0;
     for (int i = 0; i < n; ++i) columns += (int64_t)c[i].tiles_x * c[i].tiles_y * K;
-    const int64_t target = 4 * (int64_t)grid;
+    const int64_t target = (int64_t)per_block * grid;
     const int64_t want = columns >= target ? 1 : (target + columns - 1) / columns;
     int64_t items = 0;
     for (int i = 0; i < n; ++i) {
@@ -1359,6 +1360,72 @@ static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_
     }
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
                        rec, ratio);
+}
+
+int march_max_grid(int k)
+{
+    switch (k) {
+    case 1: return march_grid<1>();   case 2: return march_grid<2>();   case 3: return march_grid<3>();
+    case 4: return march_grid<4>();   case 5: return march_grid<5>();   case 6: return march_grid<6>();
+    case 7: return march_grid<7>();   case 8: return march_grid<8>();   case 9: return march_grid<9>();
+    case 10: return march_grid<10>(); case 11: return march_grid<11>(); case 12: return march_grid<12>();
+    case 13: return march_grid<13>(); case 14: return march_grid<14>(); case 15: return march_grid<15>();
+    case 16: return march_grid<16>(); default: return 0;
+    }
+}
+
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int* nitems)
+{
+    int g = 0;
+    switch (k) {
+    case 1: g = march_shape<1>(c, n, per_block, grid_limit); break;
+    case 2: g = march_shape<2>(c, n, per_block, grid_limit); break;
+    case 3: g = march_shape<3>(c, n, per_block, grid_limit); break;
+    case 4: g = march_shape<4>(c, n, per_block, grid_limit); break;
+    case 5: g = march_shape<5>(c, n, per_block, grid_limit); break;
+    case 6: g = march_shape<6>(c, n, per_block, grid_limit); break;
+    case 7: g = march_shape<7>(c, n, per_block, grid_limit); break;
+    case 8: g = march_shape<8>(c, n, per_block, grid_limit); break;
+    case 9: g = march_shape<9>(c, n, per_block, grid_limit); break;
+    case 10: g = march_shape<10>(c, n, per_block, grid_limit); break;
+    case 11: g = march_shape<11>(c, n, per_block, grid_limit); break;
+    case 12: g = march_shape<12>(c, n, per_block, grid_limit); break;
+    case 13: g = march_shape<13>(c, n, per_block, grid_limit); break;
+    case 14: g = march_shape<14>(c, n, per_block, grid_limit); break;
+    case 15: g = march_shape<15>(c, n, per_block, grid_limit); break;
+    case 16: g = march_shape<16>(c, n, per_block, grid_limit); break;
+    default: return 0;
+    }
+    int64_t items = 0;
+    for (int i = 0; i < n; ++i) items += (int64_t)c[i].tiles_x * c[i].tiles_y * k * c[i].nseg;
+    if (nitems) *nitems = (int)items;
+    return g;
+}
+
+hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems)
+{
+    JfaClip none{};
+    const dim3 g((unsigned)grid), b(256);
+    switch (k) {
+    case 1: hipLaunchKernelGGL(k_jfa_march<1>, g, b, 0, st, none, dev, n, nitems); break;
+    case 2: hipLaunchKernelGGL(k_jfa_march<2>, g, b, 0, st, none, dev, n, nitems); break;
+    case 3: hipLaunchKernelGGL(k_jfa_march<3>, g, b, 0, st, none, dev, n, nitems); break;
+    case 4: hipLaunchKernelGGL(k_jfa_march<4>, g, b, 0, st, none, dev, n, nitems); break;
+    case 5: hipLaunchKernelGGL(k_jfa_march<5>, g, b, 0, st, none, dev, n, nitems); break;
+    case 6: hipLaunchKernelGGL(k_jfa_march<6>, g, b, 0, st, none, dev, n, nitems); break;
+    case 7: hipLaunchKernelGGL(k_jfa_march<7>, g, b, 0, st, none, dev, n, nitems); break;
+    case 8: hipLaunchKernelGGL(k_jfa_march<8>, g, b, 0, st, none, dev, n, nitems); break;
+    case 9: hipLaunchKernelGGL(k_jfa_march<9>, g, b, 0, st, none, dev, n, nitems); break;
+    case 10: hipLaunchKernelGGL(k_jfa_march<10>, g, b, 0, st, none, dev, n, nitems); break;
+    case 11: hipLaunchKernelGGL(k_jfa_march<11>, g, b, 0, st, none, dev, n, nitems); break;
+    case 12: hipLaunchKernelGGL(k_jfa_march<12>, g, b, 0, st, none, dev, n, nitems); break;
+    case 13: hipLaunchKernelGGL(k_jfa_march<13>, g, b, 0, st, none, dev, n, nitems); break;
+    case 14: hipLaunchKernelGGL(k_jfa_march<14>, g, b, 0, st, none, dev, n, nitems); break;
+    case 15: hipLaunchKernelGGL(k_jfa_march<15>, g, b, 0, st, none, dev, n, nitems); break;
+    case 16: hipLaunchKernelGGL(k_jfa_march<16>, g, b, 0, st, none, dev, n, nitems); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,

@@ -277,6 +277,37 @@ int asv_stream_run_video(const asv_stream_params* p, const uint8_t* rgb, const f
 const char* asv_stream_last_error(void);
 
 /* ---- Served streaming ------------------------------------------------- */
+/* Batched clips: several clips (uneven sizes, one Lloyd iteration count)
+ * run their loops together, the batched form of Render::run
+ * (GPU-AS/Render.cpp:254-319) for many clips.  Every JFA pass index is one
+ * persistent launch over the (clip, column) work items of all clips sharing
+ * its step; each Lloyd iteration is captured once as a HIP graph and
+ * replayed; an autotuner picks each launch's persistent block count and
+ * work-item granularity on the first run.  Labels and sums of every clip are
+ * bit-identical to a single-clip asv_run.  No counterpart in the reference
+ * (one clip per process there); serves callers with many clips. */
+typedef struct asv_batch asv_batch;
+/* n clips with parameters params[0..n-1] (equal lloyd_iters); the batch
+ * owns one context per clip. */
+int asv_batch_create(const asv_params* params, int32_t n, asv_batch** out);
+int asv_batch_destroy(asv_batch* b);
+int asv_batch_size(const asv_batch* b);
+/* Context of clip i, for asv_upload_frames / asv_upload_flow before a run and
+ * asv_download_labels / asv_download_sums after it (do not destroy it). */
+int asv_batch_clip(asv_batch* b, int32_t i, asv_ctx** ctx);
+/* LAB + metric for every clip, then all Lloyd iterations, ordered after
+ * prior work on stream; returns without waiting. */
+int asv_batch_run(asv_batch* b, void* stream);
+/* 0 turns the autotuner off (default shapes); before the first run. */
+int asv_batch_set_autotune(asv_batch* b, int32_t on);
+/* Launch groups per iteration (JFA launches: batched and per-clip). */
+int asv_batch_launch_count(const asv_batch* b, int32_t* per_iteration);
+/* Launch shapes in use: per group {pass index, step K, clips (-1: one
+ * clip's own pass), grid, work items per block}; returns the group count;
+ * *tune_ms = GPU time the autotuner spent. */
+int asv_batch_tuning(const asv_batch* b, int32_t* shapes, int32_t cap, float* tune_ms);
+const char* asv_batch_last_error(void);
+
 /* No single counterpart in the reference (its streaming program handles one
  * video, streamGPU-AS/Render.cpp:100-300); the served loop runs many streams'
  * temporal blocks concurrently on one GPU with admission by footprint. */
