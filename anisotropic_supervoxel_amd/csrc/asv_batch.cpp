@@ -56,7 +56,8 @@ struct PassGroup {
     std::vector<JfaClip> desc;     // host copy (seg, nseg, item_begin planned)
     size_t dev_off = 0;            // offset of desc in the device array
     int grid = 0, nitems = 0;
-    int per_block = 4, grid_limit = 0;
+    int per_block = 4, grid_limit = 0, seg_cap = 0;
+    bool split = false;            // tuned: the clips' own launches are faster
 };
 
 JfaClip descriptor(const asv_ctx* c, int p)
@@ -110,7 +111,7 @@ namespace {
 
 int upload_group(asv_batch* b, PassGroup& g)
 {
-    g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), g.per_block, g.grid_limit, &g.nitems);
+    g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), g.per_block, g.grid_limit, g.seg_cap, &g.nitems);
     if (g.grid <= 0) return bfail(ASV_ERR_ARG, "no marching kernel for this step");
     BHIP(hipMemcpy(b->dev_desc + g.dev_off, g.desc.data(), sizeof(JfaClip) * g.desc.size(), hipMemcpyHostToDevice));
     return ASV_OK;
@@ -118,13 +119,15 @@ int upload_group(asv_batch* b, PassGroup& g)
 
 int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
 {
-    if (g.batched) {
+    if (g.batched && !g.split) {
         BHIP(launch_march_batch(st, g.K, b->dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems));
         return ASV_OK;
     }
-    const asv_ctx* c = b->clips[(size_t)g.clips[0]];
-    BHIP(launch_jfa(st, c->d, c->labels[g.p & 1], c->labels[1 - (g.p & 1)], c->lab, c->rec, g.K,
-                    c->mult[(size_t)g.p] * c->ratio, c->p.tie_rule));
+    for (int i : g.clips) {
+        const asv_ctx* c = b->clips[(size_t)i];
+        BHIP(launch_jfa(st, c->d, c->labels[g.p & 1], c->labels[1 - (g.p & 1)], c->lab, c->rec, g.K,
+                        c->mult[(size_t)g.p] * c->ratio, c->p.tie_rule));
+    }
     return ASV_OK;
 }
 
@@ -144,9 +147,10 @@ int enqueue_iteration(asv_batch* b, int iter, hipStream_t st)
     return ASV_OK;
 }
 
-// Times each batched group's candidate launch shapes on iteration 1's own
-// labels (every pass reads the previous pass's output), keeps the fastest,
-// and leaves the group planned with it.
+// Times each batched group's candidate launch shapes (grid limit x work
+// items per block x slices per segment, and the clips' own launches) on
+// iteration 1's own labels (every pass reads the previous pass's output),
+// keeps the fastest, and leaves the group planned with it.
 int autotune(asv_batch* b)
 {
     hipStream_t st = b->st;
@@ -160,32 +164,46 @@ int autotune(asv_batch* b)
     for (PassGroup& g : b->groups) {
         if (g.batched) {
             const int gmax = march_max_grid(g.K);
-            const int limits[3] = {0, (gmax * 3 / 4 / 8) * 8, (gmax / 2 / 8) * 8};
-            const int per_blocks[3] = {2, 4, 8};
+            struct Shape {
+                int gl, pb, cap;
+                bool split;
+            };
+            std::vector<Shape> cand;
+            for (int gl : {0, (gmax * 3 / 4 / 8) * 8})
+                for (int pb : {2, 4, 8})
+                    for (int cap : {0, 32, 12}) cand.push_back({gl, pb, cap, false});
+            if (g.clips.size() > 1) cand.push_back({0, 4, 0, true});
+            auto time_shape = [&](const Shape& sh, float* ms) -> int {
+                g.grid_limit = sh.gl;
+                g.per_block = sh.pb;
+                g.seg_cap = sh.cap;
+                g.split = sh.split;
+                int r;
+                if (!sh.split && (r = upload_group(b, g))) return r;
+                if ((r = launch_group(b, g, st))) return r;             // warm
+                BHIP(hipEventRecord(e0, st));
+                for (int k = 0; k < 3; ++k)
+                    if ((r = launch_group(b, g, st))) return r;
+                BHIP(hipEventRecord(e1, st));
+                BHIP(hipEventSynchronize(e1));
+                BHIP(hipEventElapsedTime(ms, e0, e1));
+                return ASV_OK;
+            };
             float best = 1e30f;
-            int best_pb = 4, best_gl = 0;
-            for (int gl : limits)
-                for (int pb : per_blocks) {
-                    g.per_block = pb;
-                    g.grid_limit = gl;
-                    if ((rc = upload_group(b, g))) return rc;
-                    if ((rc = launch_group(b, g, st))) return rc;       // warm
-                    BHIP(hipEventRecord(e0, st));
-                    for (int r = 0; r < 3; ++r)
-                        if ((rc = launch_group(b, g, st))) return rc;
-                    BHIP(hipEventRecord(e1, st));
-                    BHIP(hipEventSynchronize(e1));
-                    float ms = 0.0f;
-                    BHIP(hipEventElapsedTime(&ms, e0, e1));
-                    total += ms;
-                    if (ms < best) {
-                        best = ms;
-                        best_pb = pb;
-                        best_gl = gl;
-                    }
+            Shape bs = cand[0];
+            for (const Shape& sh : cand) {
+                float ms = 0.0f;
+                if ((rc = time_shape(sh, &ms))) return rc;
+                total += ms;
+                if (ms < best) {
+                    best = ms;
+                    bs = sh;
                 }
-            g.per_block = best_pb;
-            g.grid_limit = best_gl;
+            }
+            g.grid_limit = bs.gl;
+            g.per_block = bs.pb;
+            g.seg_cap = bs.cap;
+            g.split = bs.split;
             if ((rc = upload_group(b, g))) return rc;
         }
         if ((rc = launch_group(b, g, st))) return rc;                   // the pass's real output
@@ -348,7 +366,7 @@ int asv_batch_tuning(const asv_batch* b, int32_t* shapes, int32_t cap, float* tu
         if (shapes && n < cap) {
             shapes[5 * n + 0] = g.p;
             shapes[5 * n + 1] = g.K;
-            shapes[5 * n + 2] = g.batched ? (int)g.desc.size() : -1;
+            shapes[5 * n + 2] = g.batched && !g.split ? (int)g.desc.size() : -(int)g.clips.size();
             shapes[5 * n + 3] = g.grid;
             shapes[5 * n + 4] = g.per_block;
         }

@@ -58,11 +58,12 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 // The marching pass over several clips at once (step k <= 16, lowest-id
 // tie, A <= kMarchMaxIds per clip).  march_plan fills seg, nseg and
 // item_begin of the n host descriptors for about per_block work items per
-// resident block (grid_limit > 0 caps the grid) and returns the grid size;
-// the descriptors then go to device memory for launch_march_batch.
+// resident block (grid_limit > 0 caps the grid, seg_cap > 0 the slices per
+// segment) and returns the grid size; the descriptors then go to device
+// memory for launch_march_batch.
 constexpr int kMarchMaxIds = 16384;
 int march_max_grid(int k);
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int* nitems);
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems);
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,

@@ -842,16 +842,24 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 
     int known = 0;                            // s_count as of the last barrier
 
+    // a block's items increase, so its clip index only moves forward: the
+    // descriptor is reloaded only when the item leaves the current clip
+    int ci = -1, ci_end = 0;
+    JfaClip c = one;
     for (int item = i_beg + slot_in_xcd; item < i_end; item += per_xcd) {
-        JfaClip c = one;
-        if (clips) {
-            int lo = 0, hi = nclips - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (clips[mid].item_begin <= item) lo = mid;
-                else hi = mid - 1;
+        if (clips && item >= ci_end) {
+            if (ci < 0) {
+                int lo = 0, hi = nclips - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (clips[mid].item_begin <= item) lo = mid;
+                    else hi = mid - 1;
+                }
+                ci = lo;
             }
-            c = clips[lo];
+            while (ci + 1 < nclips && clips[ci + 1].item_begin <= item) ++ci;
+            c = clips[ci];
+            ci_end = ci + 1 < nclips ? clips[ci + 1].item_begin : nitems;
         }
         const int W = c.W, H = c.H, D = c.D;
         // the descriptor's pointers are global memory (flat accesses would
@@ -1322,7 +1330,7 @@ static int march_grid()
 // segments of its ceil(D / K) slices.  grid_limit (> 0) caps the resident
 // blocks.  Fills seg, nseg and item_begin; returns the grid size.
 template <int K>
-static int march_shape(JfaClip* c, int n, int per_block = 4, int grid_limit = 0)
+static int march_shape(JfaClip* c, int n, int per_block = 4, int grid_limit = 0, int seg_cap = 0)
 {
     int grid = march_grid<K>();
     if (grid_limit > 0 && grid_limit < grid) grid = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
@@ -1335,6 +1343,7 @@ static int march_shape(JfaClip* c, int n, int per_block = 4, int grid_limit = 0)
         const int steps = (c[i].D + K - 1) / K;
         int nseg = (int)(want < steps ? want : steps);
         if (nseg < 1) nseg = 1;
+        if (seg_cap > 0 && (steps + nseg - 1) / nseg > seg_cap) nseg = (steps + seg_cap - 1) / seg_cap;
         c[i].seg = (steps + nseg - 1) / nseg;
         c[i].nseg = (steps + c[i].seg - 1) / c[i].seg;
         c[i].item_begin = (int)items;
@@ -1374,26 +1383,26 @@ int march_max_grid(int k)
     }
 }
 
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int* nitems)
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems)
 {
     int g = 0;
     switch (k) {
-    case 1: g = march_shape<1>(c, n, per_block, grid_limit); break;
-    case 2: g = march_shape<2>(c, n, per_block, grid_limit); break;
-    case 3: g = march_shape<3>(c, n, per_block, grid_limit); break;
-    case 4: g = march_shape<4>(c, n, per_block, grid_limit); break;
-    case 5: g = march_shape<5>(c, n, per_block, grid_limit); break;
-    case 6: g = march_shape<6>(c, n, per_block, grid_limit); break;
-    case 7: g = march_shape<7>(c, n, per_block, grid_limit); break;
-    case 8: g = march_shape<8>(c, n, per_block, grid_limit); break;
-    case 9: g = march_shape<9>(c, n, per_block, grid_limit); break;
-    case 10: g = march_shape<10>(c, n, per_block, grid_limit); break;
-    case 11: g = march_shape<11>(c, n, per_block, grid_limit); break;
-    case 12: g = march_shape<12>(c, n, per_block, grid_limit); break;
-    case 13: g = march_shape<13>(c, n, per_block, grid_limit); break;
-    case 14: g = march_shape<14>(c, n, per_block, grid_limit); break;
-    case 15: g = march_shape<15>(c, n, per_block, grid_limit); break;
-    case 16: g = march_shape<16>(c, n, per_block, grid_limit); break;
+    case 1: g = march_shape<1>(c, n, per_block, grid_limit, seg_cap); break;
+    case 2: g = march_shape<2>(c, n, per_block, grid_limit, seg_cap); break;
+    case 3: g = march_shape<3>(c, n, per_block, grid_limit, seg_cap); break;
+    case 4: g = march_shape<4>(c, n, per_block, grid_limit, seg_cap); break;
+    case 5: g = march_shape<5>(c, n, per_block, grid_limit, seg_cap); break;
+    case 6: g = march_shape<6>(c, n, per_block, grid_limit, seg_cap); break;
+    case 7: g = march_shape<7>(c, n, per_block, grid_limit, seg_cap); break;
+    case 8: g = march_shape<8>(c, n, per_block, grid_limit, seg_cap); break;
+    case 9: g = march_shape<9>(c, n, per_block, grid_limit, seg_cap); break;
+    case 10: g = march_shape<10>(c, n, per_block, grid_limit, seg_cap); break;
+    case 11: g = march_shape<11>(c, n, per_block, grid_limit, seg_cap); break;
+    case 12: g = march_shape<12>(c, n, per_block, grid_limit, seg_cap); break;
+    case 13: g = march_shape<13>(c, n, per_block, grid_limit, seg_cap); break;
+    case 14: g = march_shape<14>(c, n, per_block, grid_limit, seg_cap); break;
+    case 15: g = march_shape<15>(c, n, per_block, grid_limit, seg_cap); break;
+    case 16: g = march_shape<16>(c, n, per_block, grid_limit, seg_cap); break;
     default: return 0;
     }
     int64_t items = 0;

@@ -15,6 +15,8 @@ pass's average launch time from HIP events on the launch stream.
 `served` is the streaming form: the plain per-window loop against the
 served loop with continuous batching over concurrent streams, aggregate
 Mvoxel-iters/s and p50/p99 window latency for each (see served()).
+`batched` is 64 clips of uneven sizes run together against the plain
+per-clip loop on the same resident contexts (see batched()).
 """
 import argparse
 import json
@@ -110,6 +112,59 @@ def served(n_streams, actives, seed):
     }
 
 
+def batched(n_clips, steps):
+    """Batched workload: n_clips clips from the generator (video seeds
+    100.., sizes {160x120, 320x240, 480x360, 640x480}, 20-120 frames,
+    K 200-4000, 8-32 ellipses, 10 Lloyd iterations; see batch.py), resident
+    on the device.  Times `steps` whole runs (LAB + metric and 10 iterations
+    of every clip) of the plain per-clip loop (asv_run clip after clip on the
+    same contexts) and of the batched form (asv_batch_run: graphs, batched
+    marching launches, autotuned shapes; tuning happens in an untimed first
+    run), and checks every clip's final sums agree between the two."""
+    import numpy as np
+    import torch
+    from anisotropic_supervoxel_amd import batch as B
+    clips = B.make_clips(n_clips)
+    b = B.Batch([B.clip_params(c) for c in clips])
+    vox = 0
+    for i, c in enumerate(clips):
+        rgb, fwd, bwd, _ = B.clip_video(c)
+        b.upload(i, rgb, fwd, bwd)
+        vox += c["W"] * c["H"] * c["D"]
+    torch.cuda.synchronize()
+    iters = clips[0]["iters"]
+
+    def plain():
+        for cl in b.clips:
+            cl.run()
+
+    out = {}
+    for name, fn in (("plain_per_clip_loop", plain), ("batched", b.run)):
+        fn()                                   # warm-up (and, batched, autotune + capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_run": round(dt * 1e3, 2), "mvoxel_iters_per_s": round(vox * iters / dt / 1e6, 1)}
+        out[name + "_sums"] = [b.sums(i) for i in range(n_clips)]
+    same = all(np.array_equal(a, c) for a, c in zip(out.pop("plain_per_clip_loop_sums"), out.pop("batched_sums")))
+    shapes, tune_ms = b.tuning()
+    b.close()
+    return {
+        "workload": f"{n_clips} clips, video seeds 100-{99 + n_clips}, 160x120/320x240/480x360/640x480, "
+                    "20-120 frames, K 200-4000, 8-32 ellipses, 10 Lloyd iterations",
+        "voxels": int(vox),
+        **out,
+        "batched_over_plain": round(out["batched"]["mvoxel_iters_per_s"] / out["plain_per_clip_loop"]["mvoxel_iters_per_s"], 3),
+        "sums_identical": bool(same),
+        "jfa_launches_per_iteration": len(shapes),
+        "batched_launches": sum(1 for s in shapes if s[2] > 0),
+        "autotune_gpu_ms": round(tune_ms, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +176,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-streams", type=int, default=24)
     ap.add_argument("--served-active", default="4,8", help="comma-separated concurrent-stream limits")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-clips", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB torch copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -209,8 +267,11 @@ def main():
         },
         "cpu_baseline": None,
         "served": None,
+        "batched": None,
     }
     clip.close()
+    if not args.no_batched:
+        out["batched"] = batched(args.batched_clips, args.batched_steps)
     if not args.no_served:
         out["served"] = served(args.served_streams, [int(a) for a in args.served_active.split(",")], seed=5)
     if not args.no_cpu_baseline:

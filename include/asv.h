@@ -302,9 +302,10 @@ int asv_batch_run(asv_batch* b, void* stream);
 int asv_batch_set_autotune(asv_batch* b, int32_t on);
 /* Launch groups per iteration (JFA launches: batched and per-clip). */
 int asv_batch_launch_count(const asv_batch* b, int32_t* per_iteration);
-/* Launch shapes in use: per group {pass index, step K, clips (-1: one
- * clip's own pass), grid, work items per block}; returns the group count;
- * *tune_ms = GPU time the autotuner spent. */
+/* Launch shapes in use: per group {pass index, step K, clips in the one
+ * launch (or minus the number of clips launched one by one), grid, work
+ * items per block}; returns the group count; *tune_ms = GPU time the
+ * autotuner spent. */
 int asv_batch_tuning(const asv_batch* b, int32_t* shapes, int32_t cap, float* tune_ms);
 const char* asv_batch_last_error(void);
 

@@ -53,11 +53,12 @@ def test_batch_equals_single_clip_runs(have_gpu, autotune):
     try:
         for i, (p, rgb, fwd, bwd) in enumerate(ins):
             b.upload(i, rgb, fwd, bwd)
-        shapes, _ = b.tuning()
-        assert any(s[2] > 1 for s in shapes)                  # some launch covers several clips
-        assert any(s[2] == -1 for s in shapes)                # and the per-clip fallback is exercised
         for rep in range(2):                                  # second run replays the captured graphs
             b.run()
+            shapes, _ = b.tuning()
+            if not autotune:
+                assert any(s[2] > 1 for s in shapes)          # launches covering several clips
+            assert any(s[2] == -1 for s in shapes)            # the per-clip fallback (step 18)
             for i in range(len(ins)):
                 assert np.array_equal(b.labels(i), want[i][0]), (rep, i)
                 assert np.array_equal(b.sums(i), want[i][1]), (rep, i)
