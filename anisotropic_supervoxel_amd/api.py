@@ -254,7 +254,7 @@ class Clip:
         check(self.lib.asv_run(self.h, self.stream))
 
     def run_timed(self):
-        ms = np.zeros(5, np.float32)
+        ms = np.zeros(6, np.float32)
         n = ctypes.c_int32(0)
         check(self.lib.asv_run_timed(self.h, self.stream, ptr(ms), ctypes.byref(n)))
         return ms, n.value

@@ -55,8 +55,8 @@ struct PassGroup {
     std::vector<int> clips;
     std::vector<JfaClip> desc;     // host copy (seg, nseg, item_begin planned)
     size_t dev_off = 0;            // offset of desc in the device array
-    int grid = 0, nitems = 0;
-    int per_block = 4, grid_limit = 0, seg_cap = 0;
+    int grid = 0, nitems = 0, dyn = 0;
+    int per_block = 0, grid_limit = 0, seg_cap = 0;   // per_block 0: planned by predicted time
     bool split = false;            // tuned: the clips' own launches are faster
 };
 
@@ -111,7 +111,8 @@ namespace {
 
 int upload_group(asv_batch* b, PassGroup& g)
 {
-    g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), g.per_block, g.grid_limit, g.seg_cap, &g.nitems);
+    g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), g.per_block, g.grid_limit, g.seg_cap, &g.nitems,
+                        &g.dyn);
     if (g.grid <= 0) return bfail(ASV_ERR_ARG, "no marching kernel for this step");
     BHIP(hipMemcpy(b->dev_desc + g.dev_off, g.desc.data(), sizeof(JfaClip) * g.desc.size(), hipMemcpyHostToDevice));
     return ASV_OK;
@@ -120,13 +121,18 @@ int upload_group(asv_batch* b, PassGroup& g)
 int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
 {
     if (g.batched && !g.split) {
-        BHIP(launch_march_batch(st, g.K, b->dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems));
+        BHIP(launch_march_batch(st, g.K, b->dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn));
         return ASV_OK;
     }
     for (int i : g.clips) {
         const asv_ctx* c = b->clips[(size_t)i];
-        BHIP(launch_jfa(st, c->d, c->labels[g.p & 1], c->labels[1 - (g.p & 1)], c->lab, c->rec, g.K,
-                        c->mult[(size_t)g.p] * c->ratio, c->p.tie_rule));
+        const float ratio = c->mult[(size_t)g.p] * c->ratio;
+        if (g.p == 0 && g.K <= kSeedPassMaxStep)      // the seed stage has just scattered the seeds
+            BHIP(launch_jfa_seeds(st, c->d, c->labels[0], c->labels[1], c->lab, c->rec, c->pos, g.K, ratio,
+                                  c->p.tie_rule));
+        else
+            BHIP(launch_jfa(st, c->d, c->labels[g.p & 1], c->labels[1 - (g.p & 1)], c->lab, c->rec, g.K, ratio,
+                            c->p.tie_rule));
     }
     return ASV_OK;
 }
@@ -141,6 +147,7 @@ int enqueue_iteration(asv_batch* b, int iter, hipStream_t st)
     for (const PassGroup& g : b->groups)
         if ((rc = launch_group(b, g, st))) return rc;
     for (asv_ctx* c : b->clips) {
+        c->fresh = false;
         c->cur = (int)(c->steps.size() & 1);
         if ((rc = asv_centroids(c, iter, st))) return rc;
     }
@@ -163,16 +170,16 @@ int autotune(asv_batch* b)
     float total = 0.0f;
     for (PassGroup& g : b->groups) {
         if (g.batched) {
-            const int gmax = march_max_grid(g.K);
+            const int gmax = march_max_grid(g.K, g.dyn);
             struct Shape {
                 int gl, pb, cap;
                 bool split;
             };
             std::vector<Shape> cand;
             for (int gl : {0, (gmax * 3 / 4 / 8) * 8})
-                for (int pb : {2, 4, 8})
+                for (int pb : {0, 2, 4, 8})
                     for (int cap : {0, 32, 12}) cand.push_back({gl, pb, cap, false});
-            if (g.clips.size() > 1) cand.push_back({0, 4, 0, true});
+            if (g.clips.size() > 1) cand.push_back({0, 0, 0, true});
             auto time_shape = [&](const Shape& sh, float* ms) -> int {
                 g.grid_limit = sh.gl;
                 g.per_block = sh.pb;
@@ -247,7 +254,8 @@ int asv_batch_create(const asv_params* params, int32_t n, asv_batch** out)
             const asv_ctx* c = b->clips[(size_t)i];
             if (p >= (int)c->steps.size()) continue;
             const int K = c->steps[(size_t)p];
-            const bool march = K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID && c->d.A <= kMarchMaxIds;
+            // pass 0 runs per clip over the scattered seeds only
+            const bool march = p > 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID && c->d.A <= kMarchMaxIds;
             PassGroup* g = nullptr;
             if (march)
                 for (PassGroup& q : row)

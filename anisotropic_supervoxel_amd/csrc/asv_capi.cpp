@@ -490,9 +490,27 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
     HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->field, c->fwd, gate, c->rec, tr));
     HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
     HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
+    c->fresh = true;
     c->last_iter = iter;
     return ASV_OK;
 }
+
+namespace {
+
+// Pass i of the schedule from labels[cur] into labels[1 - cur].  The first
+// pass after the seed stage sees only the scattered seeds and, for steps up
+// to kSeedPassMaxStep, evaluates just the voxels around them.
+hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
+{
+    const float ratio = c->mult[i] * c->ratio;
+    if (i == 0 && c->fresh && c->steps[0] <= kSeedPassMaxStep)
+        return launch_jfa_seeds(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->pos,
+                                c->steps[0], ratio, c->p.tie_rule);
+    return launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i], ratio,
+                      c->p.tie_rule);
+}
+
+}  // namespace
 
 int asv_flood(asv_ctx* c, int32_t iter, void* stream)
 {
@@ -500,10 +518,10 @@ int asv_flood(asv_ctx* c, int32_t iter, void* stream)
     if (iter != c->last_iter) return fail(ASV_ERR_STATE, "flood before the seed stage of this iteration");
     hipStream_t st = S(stream);
     for (size_t i = 0; i < c->steps.size(); ++i) {
-        HIP_TRY(launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i],
-                           c->mult[i] * c->ratio, c->p.tie_rule));
+        HIP_TRY(launch_pass(c, i, st));
         c->cur = 1 - c->cur;
     }
+    c->fresh = false;
     return ASV_OK;
 }
 
@@ -567,12 +585,13 @@ int asv_run_timed(asv_ctx* c, void* stream, float* ms, int32_t* passes)
         spans.push_back({a, mark(), 4});
         for (size_t i = 0; i < c->steps.size(); ++i) {
             a = mark();
-            HIP_TRY(launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i],
-                               c->mult[i] * c->ratio, c->p.tie_rule));
-            spans.push_back({a, mark(), 2});
+            const bool first = i == 0 && c->fresh && c->steps[0] <= kSeedPassMaxStep;
+            HIP_TRY(launch_pass(c, i, st));
+            spans.push_back({a, mark(), first ? 5 : 2});
             c->cur = 1 - c->cur;
-            ++np;
+            if (!first) ++np;
         }
+        c->fresh = false;
         unsigned long long* s = c->sums[it & 1];
         a = mark();
         HIP_TRY(launch_reduce(st, c->d, c->labels[c->cur], c->lab, s));
@@ -585,7 +604,7 @@ int asv_run_timed(asv_ctx* c, void* stream, float* ms, int32_t* passes)
     }
     hipEvent_t t1 = mark();
     HIP_TRY(hipEventSynchronize(t1));
-    for (int i = 0; i < 5; ++i) ms[i] = 0.0f;
+    for (int i = 0; i < 6; ++i) ms[i] = 0.0f;
     float f = 0.0f;
     HIP_TRY(hipEventElapsedTime(&f, t0, t1));
     ms[0] = f;

@@ -20,6 +20,7 @@ struct asv_ctx {
     bool have_frames = false, have_flow = false, prepared = false;
     bool trace_on = false;
     int cur = 0;            // index of the label buffer holding the latest labels
+    bool fresh = false;     // labels[cur] holds only the seed stage's scattered seeds
     int last_iter = 0;
 
     uint8_t* rgb = nullptr;

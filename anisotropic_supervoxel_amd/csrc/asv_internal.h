@@ -17,7 +17,7 @@ struct alignas(16) SeedRec {
 
 // One clip's share of a JFA pass launch (the marching kernel covers one or
 // several clips; work items item_begin .. of this clip are its 16 x 16
-// columns x K residues x nseg segments of seg slices).
+// columns x K residues x nseg segments of up to seg slices).
 struct JfaClip {
     const int32_t* in;
     int32_t* out;
@@ -55,16 +55,27 @@ hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
 hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule);
+// The first pass of an iteration when `in` holds only the seeds scattered
+// by the seed stage at `pos` (every other voxel -1): fills `out` with -1
+// and evaluates only the voxels that have a seed among their taps, at most
+// max(3, k + 1)^3 per seed; used for steps k <= kSeedPassMaxStep.
+constexpr int kSeedPassMaxStep = 16;
+hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule);
 // The marching pass over several clips at once (step k <= 16, lowest-id
 // tie, A <= kMarchMaxIds per clip).  march_plan fills seg, nseg and
-// item_begin of the n host descriptors for about per_block work items per
-// resident block (grid_limit > 0 caps the grid, seg_cap > 0 the slices per
-// segment) and returns the grid size; the descriptors then go to device
-// memory for launch_march_batch.
+// item_begin of the n host descriptors (per_block > 0: about per_block
+// items per resident block; 0: the segment count with the least predicted
+// time; grid_limit > 0 caps the grid, seg_cap > 0 the slices per segment),
+// sets the launch's item count and dynamic LDS bytes, and returns the grid
+// size (0: the marching pass does not apply); the descriptors then go to
+// device memory for launch_march_batch.  march_max_grid is the resident
+// block count for `dyn` dynamic LDS bytes.
 constexpr int kMarchMaxIds = 16384;
-int march_max_grid(int k);
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems);
-hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems);
+int march_mc(int k);
+int march_max_grid(int k, int dyn);
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn);
+hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums);

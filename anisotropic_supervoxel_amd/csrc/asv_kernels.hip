@@ -18,6 +18,9 @@
 
 #include "asv_internal.h"
 
+#include <cstdlib>
+
+
 namespace asv {
 
 // A pointer into global memory (address space 1): loads and stores through
@@ -512,21 +515,29 @@ __global__ void k_place(int W, int H, int A, const int4* __restrict__ pos, int32
 // JFA pass (flood.gs:53-209)
 // ---------------------------------------------------------------------------
 
-// Distance of voxel (fx, fy, fz) with colour (l0, l1, l2) to one seed record,
-// in the fixed fused order of the specification:
-//   t1 = fma(dz, 2m2, fma(dy, 2m1, dx*m0)),  t2 = fma(dz, 2m4, dy*m3)
-//   q  = max(0, fma(dx, t1, fma(dy, t2, (dz*dz)*m5)))
+// The non-negative part of a quadratic form value: a value with its sign
+// bit set (negative, -0) becomes +0, any other value (NaN included) passes.
+__device__ __forceinline__ float nonneg(float q) { return __int_as_float(max(__float_as_int(q), 0)); }
+
+// Distance of voxel (fx, fy, fz) with colour (l0, l1, l2) to one seed record
+// (a = (x, y, z, m0), b = (2m1, 2m2, m3, 2m4), c = (m5, L, a, b)), in the
+// fixed order of the specification:
+//   u1 = dz*2m2,  u2 = dz*2m4,  u3 = (dz*dz)*m5     (rounded products)
+//   t1 = fma(dx, m0, fma(dy, 2m1, u1)),  t2 = fma(dy, m3, u2)
+//   q  = nonneg(fma(dx, t1, fma(dy, t2, u3)))
 //   d  = fma(ratio, q, fma(c2, c2, fma(c1, c1, c0*c0)))
+// The u terms depend on the seed and the slice only, so a block working on
+// one slice can compute them once per seed.
 __device__ __forceinline__ float cand_dist(const float4 a, const float4 b, const float4 c, float fx, float fy,
                                            float fz, float l0, float l1, float l2, float ratio)
 {
-    float dx = fx - a.x, dy = fy - a.y, dz = fz - a.z;
-    float t1 = __fmaf_rn(dz, b.y, __fmaf_rn(dy, b.x, dx * a.w));
-    float t2 = __fmaf_rn(dz, b.w, dy * b.z);
-    float q = __fmaf_rn(dx, t1, __fmaf_rn(dy, t2, (dz * dz) * c.x));
-    q = q < 0.0f ? 0.0f : q;
-    float c0 = l0 - c.y, c1 = l1 - c.z, c2 = l2 - c.w;
-    float cd = __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0));
+    const float dx = fx - a.x, dy = fy - a.y, dz = fz - a.z;
+    const float u1 = dz * b.y, u2 = dz * b.w, u3 = (dz * dz) * c.x;
+    const float t1 = __fmaf_rn(dx, a.w, __fmaf_rn(dy, b.x, u1));
+    const float t2 = __fmaf_rn(dy, b.z, u2);
+    const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u3)));
+    const float c0 = l0 - c.y, c1 = l1 - c.z, c2 = l2 - c.w;
+    const float cd = __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0));
     return __fmaf_rn(ratio, q, cd);
 }
 
@@ -540,29 +551,22 @@ __device__ __forceinline__ unsigned long long dkey(float d, int id)
 }
 #define ASV_KEY_INIT (((unsigned long long)0x4CBEBC20u) << 32)   /* bits of 1e8f */
 
-// Generic pass: one voxel per thread, 64 x 4 voxels per block, taps read
-// from global memory in the reference's scan order (slab z, z-k, z+k; rows
-// y-k, y, y+k; columns x-k, x, x+k).  A tap repeating one of the last two
-// evaluated ids is skipped: its distance is identical, so neither tie rule
-// can change the winner.  Serves any step and the scan-order tie rule.
+// One voxel of a pass, taps read from global memory in the reference's
+// scan order (slab z, z-k, z+k; rows y-k, y, y+k; columns x-k, x, x+k).  A
+// tap repeating one of the last two evaluated ids is skipped: its distance
+// is identical, so neither tie rule can change the winner.
 template <int TIE>
-__global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t* __restrict__ in,
-                                             int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
-                                             const SeedRec* __restrict__ rec, int k, float ratio)
+__device__ __forceinline__ int32_t jfa_voxel(int W, int H, int D, const int32_t* __restrict__ in,
+                                             const uint32_t* __restrict__ lab, const SeedRec* __restrict__ rec,
+                                             int k, float ratio, int x, int y, int z)
 {
-    int x = blockIdx.x * 64 + threadIdx.x;
-    int y = blockIdx.y * 4 + threadIdx.y;
-    int z = blockIdx.z;
-    if (x >= W || y >= H) return;
     const size_t plane = (size_t)W * H;
-    size_t v = (size_t)z * plane + (size_t)y * W + x;
-    uint32_t pk = lab[v];
-    float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
-    float fx = (float)x, fy = (float)y, fz = (float)z;
-
-    int xs[3] = {clampi(x - k, 0, W - 1), x, clampi(x + k, 0, W - 1)};
-    int ys[3] = {clampi(y - k, 0, H - 1), y, clampi(y + k, 0, H - 1)};
-    int zs[3] = {z, clampi(z - k, 0, D - 1), clampi(z + k, 0, D - 1)};
+    const uint32_t pk = lab[(size_t)z * plane + (size_t)y * W + x];
+    const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    const int xs[3] = {clampi(x - k, 0, W - 1), x, clampi(x + k, 0, W - 1)};
+    const int ys[3] = {clampi(y - k, 0, H - 1), y, clampi(y + k, 0, H - 1)};
+    const int zs[3] = {z, clampi(z - k, 0, D - 1), clampi(z + k, 0, D - 1)};
 
     unsigned long long best = ASV_KEY_INIT;
     float bestd = 100000000.0f;
@@ -575,14 +579,14 @@ __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t*
             const int32_t* row = in + (size_t)zs[sl] * plane + (size_t)ys[r] * W;
 #pragma unroll
             for (int cidx = 0; cidx < 3; ++cidx) {
-                int32_t id = row[xs[cidx]];
+                const int32_t id = row[xs[cidx]];
                 if (id < 0 || id == seen0 || id == seen1) continue;
                 seen1 = seen0;
                 seen0 = id;
-                SeedRec s = rec[id];
-                float d = cand_dist(s.a, s.b, s.c, fx, fy, fz, l0, l1, l2, ratio);
+                const SeedRec sr = rec[id];
+                const float d = cand_dist(sr.a, sr.b, sr.c, fx, fy, fz, l0, l1, l2, ratio);
                 if (TIE == 0) {
-                    unsigned long long kk = dkey(d, id);
+                    const unsigned long long kk = dkey(d, id);
                     best = kk < best ? kk : best;
                 } else if (d < bestd) {
                     bestd = d;
@@ -592,7 +596,65 @@ __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t*
         }
     }
     if (TIE == 0) best_id = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
-    out[v] = best_id;
+    return best_id;
+}
+
+// Generic pass: one voxel per thread, 64 x 4 voxels per block.  Serves any
+// step and the scan-order tie rule.
+template <int TIE>
+__global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t* __restrict__ in,
+                                             int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
+                                             const SeedRec* __restrict__ rec, int k, float ratio)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= W || y >= H) return;
+    out[((size_t)z * H + y) * W + x] = jfa_voxel<TIE>(W, H, D, in, lab, rec, k, ratio, x, y, z);
+}
+
+// The idx-th coordinate v in [0, n) (ascending) whose taps v - k, v, v + k,
+// clamped to [0, n - 1], include p; false when there are fewer.  Such v lie
+// within k of p: v = p - k o exactly, or the clamped tap at an edge p = 0
+// (v <= k) or p = n - 1 (v >= n - 1 - k).
+__device__ __forceinline__ bool seed_axis(int p, int n, int k, int idx, int* v)
+{
+    const int lo = max(0, p - k), hi = min(n - 1, p + k);
+    for (int c = lo; c <= hi; ++c) {
+        const bool hit = clampi(c - k, 0, n - 1) == p || c == p || clampi(c + k, 0, n - 1) == p;
+        if (hit && idx-- == 0) {
+            *v = c;
+            return true;
+        }
+    }
+    return false;
+}
+
+// First pass of an iteration, over a volume holding only the scattered
+// seeds (every other voxel -1, as the seed stage leaves it): a voxel finds a
+// label among its 27 clamped taps only if some seed voxel p is one of them,
+// which fixes each coordinate of the voxel to the few values seed_axis
+// enumerates (at most max(3, k + 1) per axis).  One thread per (seed, such
+// voxel) evaluates that voxel exactly as the full pass does; every other
+// voxel keeps the -1 the output was filled with.  A voxel near several seeds
+// is written by each with the same value.
+template <int TIE>
+__global__ __launch_bounds__(256) void k_jfa_seeds(int W, int H, int D, int A, int k, int m,
+                                                   const int4* __restrict__ pos, const int32_t* __restrict__ in,
+                                                   int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
+                                                   const SeedRec* __restrict__ rec, float ratio)
+{
+    const int per = m * m * m;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = gid / per;
+    if (i >= A) return;
+    const int j = (int)(gid - i * per);
+    const int4 p = pos[i];
+    int x, y, z;
+    if (!seed_axis(p.x, W, k, j % m, &x) || !seed_axis(p.y, H, k, (j / m) % m, &y) ||
+        !seed_axis(p.z, D, k, j / (m * m), &z))
+        return;
+    out[((size_t)z * H + y) * W + x] = jfa_voxel<TIE>(W, H, D, in, lab, rec, k, ratio, x, y, z);
 }
 
 // Tiled pass for the lowest-id rule and step K (template), 16 x 16 voxels of
@@ -776,64 +838,89 @@ __global__ __launch_bounds__(256) void k_jfa_tile(int W, int H, int D, int tiles
     out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
 }
 
-// LDS id table of the marching pass: one byte per seed id (A <= kDirectIds);
-// at most kListCap ids are listed per table generation.
-constexpr int kDirectIds = 16384;
+// LDS id table of the marching pass: one byte per seed id, sized at launch
+// for the largest seed count of the launch (dynamic shared memory); at most
+// kListCap ids are listed per table generation.
 constexpr int kListCap = 512;
 constexpr unsigned kSlotEmpty = 0xFFu;
 constexpr unsigned kSlotClaimed = 0xFEu;
 
-// Marching persistent pass for the lowest-id rule (A <= kDirectIds,
-// K <= 16), also the batched form: one launch can cover several clips.
-//
-// A work item is one 16 x 16 column (tx, ty) of one clip, one residue
-// r = z mod K and a segment of up to `seg` slices z = z0, z0 + K, ...
-// Moving from z to z + K keeps two of the three slabs a voxel's taps read
-// (z and z + K become the new z - K and z), so each step stages, claims
-// and converts only the new slab clamp(z + 2K):
-//  - the direct-mapped byte table slot[id] and the candidate list persist
-//    along the march (an id claimed earlier keeps its compact index); when
-//    a step would take the list past 63 candidates, the table is released
-//    and rebuilt from the three current slabs;
-//  - every thread keeps, in registers, the 9-tap candidate masks of its
-//    voxel position in the three slabs, and computes only the new slab's
-//    (9 LDS bytes); the voxel's 27-tap mask is the OR of the three;
-//  - two barriers per step (claims done; new slab and records staged).
-// A rebuild that still leaves more than 63 candidates evaluates that slice
-// per tap from global labels.
-// Work items are ordered (segment, residue, ty, tx), tx fastest, and split
-// into 8 contiguous ranges, one per XCD (block b runs on XCD b mod 8), so
-// concurrently marching columns of one XCD are neighbours that share halo
-// rows in its L2.
+// Candidate masks: MC compact indices per table generation, bit MC marks a
+// tap without a seed (or past the table).
+template <int MC> struct MarchMask;
+template <> struct MarchMask<31> { using T = uint32_t; };
+template <> struct MarchMask<63> { using T = unsigned long long; };
+
+// Slices of residue r (z = r, r + K, ...) in segment g of a clip.
 template <int K>
-__global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
-                                                   int nitems)
+__device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 {
+    if (r >= c.D) return 0;
+    const int rest = (c.D - r + K - 1) / K - g * c.seg;
+    return rest <= 0 ? 0 : (rest < c.seg ? rest : c.seg);
+}
+
+// Marching persistent pass for the lowest-id rule (K <= 16, every clip's
+// A <= kMarchMaxIds), also the batched form: one launch covers one clip
+// (`one`, clips == nullptr) or the `nclips` descriptors of `clips`.
+//
+// Work items: a clip's 16 x 16 columns (tx, ty) x residues r = z mod K x
+// segments g of up to `seg` slices z = r + K (g seg), r + K (g seg + 1), ...,
+// ordered (g, r, ty, tx), tx fastest.  The items are split into 8
+// contiguous ranges, one per XCD (block b runs on XCD b mod 8), and the
+// blocks of one XCD take its items round-robin, so concurrently marching
+// columns of an XCD are neighbours at about the same z and share halo rows
+// in its L2.  The host picks the segment count and the grid so that every
+// block of an XCD gets the same number of items (rounds).
+//
+// Moving from z to z + K keeps two of the three slabs a voxel's taps read
+// (z and z + K become the new z - K and z), so each step stages, claims and
+// converts only the new slab clamp(z + 2K):
+//  - the direct-mapped byte table slot[id] and the candidate list persist
+//    along a column (an id claimed earlier keeps its compact index); when a
+//    step would take the list past MC candidates, the table is released and
+//    rebuilt from the three current slabs;
+//  - every thread keeps, in registers, the 9-tap candidate masks of its
+//    voxel position in the three slabs, and computes only the new slab's;
+//    the voxel's 27-tap mask is the OR of the three;
+//  - the candidates' records sit in LDS as structure-of-arrays, with the
+//    three dz terms of the distance (dz 2m2, dz 2m4, dz^2 m5) recomputed
+//    once per slice for the whole block;
+//  - two barriers per step (claims done; new slab and records staged).
+// A rebuild that still leaves more than MC candidates evaluates that slice
+// per tap from global labels and records.
+template <int K, int MC>
+__global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
+                                                   int nitems, int slot_words)
+{
+    using MaskT = typename MarchMask<MC>::T;
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
     constexpr int RA = R * R;
     constexpr int RPI = 64 / R;               // staged rows per wave instruction
     constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
-    constexpr unsigned long long kBadBit = 1ull << kMaxCand;
+    constexpr MaskT kBadBit = (MaskT)1 << MC;
     constexpr bool kLabAhead = K >= 3;        // LAB of the next slice loaded during this one
-    __shared__ uint32_t s_slot[kDirectIds / 4];
+    extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
     __shared__ uint8_t s_idx[3 * RA];
-    __shared__ float4 s_rec[kMaxCand * 3];
-    __shared__ int s_rid[kMaxCand];
+    __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
+    __shared__ float4 s_u[MC];                // (dz 2m2, dz 2m4, dz^2 m5, m3) at the current slice
+    __shared__ float4 s_c[MC];                // (L, a, b, id bits)
+    __shared__ float4 s_z[MC];                // (z, 2 m2, 2 m4, m5)
     __shared__ int s_list[kListCap];
     __shared__ int s_count;
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    for (int i = tid; i < kDirectIds / 4; i += 256) s_slot[i] = 0xFFFFFFFFu;
+    for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
     if (tid == 0) s_count = 0;
     __syncthreads();
 
     const int nb = gridDim.x;                 // a multiple of 8
     const int xcd = blockIdx.x & 7, slot_in_xcd = blockIdx.x >> 3, per_xcd = nb >> 3;
-    const int q = nitems >> 3, rm = nitems & 7;
-    const int i_beg = xcd * q + min(xcd, rm);
-    const int i_end = i_beg + q + (xcd < rm ? 1 : 0);
+    const int qx = nitems >> 3, rx = nitems & 7;
+    const int i_beg = xcd * qx + min(xcd, rx);
+    const int i_end = i_beg + qx + (xcd < rx ? 1 : 0);
 
+    const int wave = tid >> 6, lane = tid & 63;
     const int sub = lane / R, col = lane - (lane / R) * R;
     const bool lane_ok = sub < RPI;
     const int lx = (wave & 1) * 8 + (lane & 7);
@@ -841,7 +928,6 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     const int mbase = (ly + K) * R + (lx + K);
 
     int known = 0;                            // s_count as of the last barrier
-
     // a block's items increase, so its clip index only moves forward: the
     // descriptor is reloaded only when the item leaves the current clip
     int ci = -1, ci_end = 0;
@@ -861,25 +947,26 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             c = clips[ci];
             ci_end = ci + 1 < nclips ? clips[ci + 1].item_begin : nitems;
         }
+        const int T = c.tiles_x * c.tiles_y;
+        const int local = item - c.item_begin;
+        const int t = local % T, gr = local / T;
+        const int r = gr % K, g = gr / K;
+        const int n = march_len<K>(c, g, r);
+        if (n <= 0) continue;
         const int W = c.W, H = c.H, D = c.D;
         // the descriptor's pointers are global memory (flat accesses would
         // share the LDS wait counter)
         const gptr<const int32_t> g_in = (gptr<const int32_t>)c.in;
         const gptr<int32_t> g_out = (gptr<int32_t>)c.out;
         const gptr<const uint32_t> g_lab = (gptr<const uint32_t>)c.lab;
-        int local = item - c.item_begin;
-        const int tx = local % c.tiles_x;
-        local /= c.tiles_x;
-        const int ty = local % c.tiles_y;
-        local /= c.tiles_y;
-        const int r = local % K;
-        const int g = local / K;
+        const gptr<const float> g_rec = (gptr<const float>)c.rec;
+        const float ratio = c.ratio;
+        const int tx = t % c.tiles_x, ty = t / c.tiles_x;
         const int x0 = tx * kTile, y0 = ty * kTile;
-        const int z_first = r + K * g * c.seg;
-        if (z_first >= D) continue;
         const int gx = clampi(x0 - K + col, 0, W - 1);
         const int x = x0 + lx, y = y0 + ly;
         const bool mine = x < W && y < H;
+        const float fx = (float)x, fy = (float)y;
 
         auto load_slab = [&](int gz, int* ids) {
             const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
@@ -903,9 +990,9 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                         const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
                         const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
                         if (prev == old) {
-                            const int j = atomicAdd(&s_count, 1);
-                            if (j < kListCap) s_list[j] = id;
-                            const unsigned byte = j < kMaxCand ? (unsigned)j : (unsigned)(kMaxCand - 1);
+                            const int k = atomicAdd(&s_count, 1);
+                            if (k < kListCap) s_list[k] = id;
+                            const unsigned byte = k < MC ? (unsigned)k : (unsigned)MC;
                             atomicAnd(&s_slot[id >> 2], ~(0xFFu << sh));
                             atomicOr(&s_slot[id >> 2], byte << sh);
                             break;
@@ -921,50 +1008,63 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 const int rr = (wave + 4 * i) * RPI + sub;
                 const int id = ids[i];
                 if (lane_ok && rr < R) {
-                    unsigned ci = (unsigned)kMaxCand;
-                    if (id >= 0) ci = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
-                    s_idx[slot * RA + rr * R + col] = (uint8_t)ci;
+                    unsigned cix = (unsigned)MC;
+                    if (id >= 0) cix = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
+                    s_idx[slot * RA + rr * R + col] = (uint8_t)cix;
                 }
             }
         };
-        auto copy_records = [&](int from, int to) {
-            for (int j = from + tid; j < to; j += 256) {
-                const int id = s_list[j];
-                const gptr<const float> q = (gptr<const float>)(c.rec + id);
-                s_rec[j * 3 + 0] = ld_f4(q);
-                s_rec[j * 3 + 1] = ld_f4(q + 4);
-                s_rec[j * 3 + 2] = ld_f4(q + 8);
-                s_rid[j] = id;
+        // records of list entries [from, to) from global memory, and the dz
+        // terms of every entry [0, to) for slice z
+        auto stage = [&](int from, int to, int z) {
+            const float fz = (float)z;
+            for (int e = tid; e < to; e += 256) {
+                float sz, m2d, m4d, m5, m3;
+                if (e >= from) {
+                    const int id = s_list[e];
+                    const gptr<const float> q = g_rec + (size_t)id * 12;
+                    const float4 a = ld_f4(q), b = ld_f4(q + 4), cc = ld_f4(q + 8);
+                    s_p[e] = make_float4(a.x, a.y, a.w, b.x);
+                    s_z[e] = make_float4(a.z, b.y, b.w, cc.x);
+                    s_c[e] = make_float4(cc.y, cc.z, cc.w, __int_as_float(id));
+                    sz = a.z; m2d = b.y; m4d = b.w; m5 = cc.x; m3 = b.z;
+                } else {
+                    const float4 zz = s_z[e];
+                    sz = zz.x; m2d = zz.y; m4d = zz.z; m5 = zz.w;
+                    m3 = s_u[e].w;
+                }
+                const float dz = fz - sz;
+                s_u[e] = make_float4(dz * m2d, dz * m4d, (dz * dz) * m5, m3);
             }
         };
         auto slab_mask = [&](int slot) {
-            unsigned long long m = 0ull;
+            MaskT m = 0;
             const uint8_t* p = s_idx + slot * RA + mbase;
 #pragma unroll
             for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) m |= 1ull << p[dy * K * R + dx * K];
+                for (int dx = -1; dx <= 1; ++dx) m |= (MaskT)1 << p[dy * K * R + dx * K];
             return m;
         };
         // release every claimed byte and empty the list (callers barrier after)
         auto release = [&]() {
             if (known <= kListCap) {
-                for (int j = tid; j < known; j += 256) {
-                    const int id = s_list[j];
+                for (int e = tid; e < known; e += 256) {
+                    const int id = s_list[e];
                     atomicOr(&s_slot[id >> 2], 0xFFu << ((unsigned)(id & 3) * 8u));
                 }
             } else {
-                for (int i = tid; i < kDirectIds / 4; i += 256) s_slot[i] = 0xFFFFFFFFu;
+                for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
             }
         };
 
-        unsigned long long m_lo = 0ull, m_mid = 0ull, m_hi = 0ull;   // slabs z-K, z, z+K
+        MaskT m_lo = 0, m_mid = 0, m_hi = 0;  // slabs z-K, z, z+K
         int ids_new[NRS];
+        int z = r + K * g * c.seg;
         uint32_t pk_next = 0u;
-        if (kLabAhead && mine) pk_next = g_lab[((size_t)z_first * H + y) * W + x];
+        if (kLabAhead && mine) pk_next = g_lab[((size_t)z * H + y) * W + x];
         bool full = true;
-        int z = z_first;
-        for (int step = 0; step < c.seg && z < D;) {
+        for (int step = 0; step < n;) {
             bool overflow = false;
             if (full) {
                 release();
@@ -980,12 +1080,12 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 claim(ids2);
                 __syncthreads();
                 known = s_count;
-                overflow = known > kMaxCand;
+                overflow = known > MC;
                 if (!overflow) {
                     convert(ids0, 0);
                     convert(ids1, 1);
                     convert(ids2, 2);
-                    copy_records(0, known);
+                    stage(0, known, z);
                 }
                 __syncthreads();
                 if (!overflow) {
@@ -998,13 +1098,13 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 claim(ids_new);
                 __syncthreads();
                 const int cnt = s_count;
-                if (cnt > kMaxCand) {         // uniform: rebuild this slice
+                if (cnt > MC) {               // uniform: rebuild this slice
                     known = cnt;
                     full = true;
                     continue;
                 }
                 convert(ids_new, 0);
-                copy_records(known, cnt);
+                stage(known, cnt, z);
                 known = cnt;
                 __syncthreads();
                 m_lo = m_mid;
@@ -1013,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             }
             // the next step's new slab (and LAB), loaded while this slice is evaluated
             const uint32_t pk_here = pk_next;
-            if (step + 1 < c.seg && z + K < D) {
+            if (step + 1 < n) {
                 load_slab(clampi(z + 2 * K, 0, D - 1), ids_new);
                 if (kLabAhead && mine) pk_next = g_lab[((size_t)(z + K) * H + y) * W + x];
             }
@@ -1022,34 +1122,43 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 const size_t v = ((size_t)z * H + y) * W + x;
                 const uint32_t pk = kLabAhead ? pk_here : g_lab[v];
                 const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
-                const float fx = (float)x, fy = (float)y, fz = (float)z;
                 unsigned long long best = ASV_KEY_INIT;
                 if (!overflow) {
-                    unsigned long long mask = (m_lo | m_mid | m_hi) & ~kBadBit;
+                    MaskT mask = (m_lo | m_mid | m_hi) & ~kBadBit;
                     while (mask) {
-                        const int j = __ffsll((long long)mask) - 1;
-                        mask &= mask - 1ull;
-                        const float d = cand_dist(s_rec[j * 3 + 0], s_rec[j * 3 + 1], s_rec[j * 3 + 2], fx, fy, fz,
-                                                  l0, l1, l2, c.ratio);
-                        const unsigned long long kk = dkey(d, s_rid[j]);
+                        const int e = MC == 31 ? __builtin_ctz((uint32_t)mask) : __builtin_ctzll((unsigned long long)mask);
+                        mask &= mask - 1;
+                        const float4 p = s_p[e], u = s_u[e], cc = s_c[e];
+                        const float dx = fx - p.x, dy = fy - p.y;
+                        const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
+                        const float t2 = __fmaf_rn(dy, u.w, u.y);
+                        const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
+                        const float c0 = l0 - cc.x, c1 = l1 - cc.y, c2 = l2 - cc.z;
+                        const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
+                        const unsigned long long kk =
+                            ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.w);
                         best = kk < best ? kk : best;
                     }
                 } else {
-                    const int xs[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
-                    const int ys[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
-                    const int zs[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
+                    const float fz = (float)z;
+                    const int xs3[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
+                    const int ys3[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
+                    const int zs3[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
                     int seen0 = -1, seen1 = -1;
+#pragma unroll 1
                     for (int sl = 0; sl < 3; ++sl)
+#pragma unroll 1
                         for (int rr = 0; rr < 3; ++rr) {
-                            const gptr<const int32_t> row = g_in + ((size_t)zs[sl] * H + ys[rr]) * W;
+                            const gptr<const int32_t> row = g_in + ((size_t)zs3[sl] * H + ys3[rr]) * W;
+#pragma unroll 1
                             for (int cc = 0; cc < 3; ++cc) {
-                                const int id = row[xs[cc]];
+                                const int id = row[xs3[cc]];
                                 if (id < 0 || id == seen0 || id == seen1) continue;
                                 seen1 = seen0;
                                 seen0 = id;
-                                const gptr<const float> q = (gptr<const float>)(c.rec + id);
+                                const gptr<const float> q = g_rec + (size_t)id * 12;
                                 const float d = cand_dist(ld_f4(q), ld_f4(q + 4), ld_f4(q + 8), fx, fy, fz, l0, l1,
-                                                          l2, c.ratio);
+                                                          l2, ratio);
                                 const unsigned long long kk = dkey(d, id);
                                 best = kk < best ? kk : best;
                             }
@@ -1060,6 +1169,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             z += K;
             ++step;
         }
+
     }
 }
 
@@ -1308,132 +1418,192 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
     return hipGetLastError();
 }
 
-// Resident blocks of k_jfa_march<K> (CUs x blocks per CU, a multiple of 8).
-template <int K>
-static int march_grid()
+// Candidate capacity of the marching pass at step k: 32-bit masks (31
+// candidates per table generation) or 64-bit masks (63).  Small steps meet
+// few distinct seeds along a column; the environment variable ASV_MARCH_MC
+// (31 or 63) overrides the choice for every step.
+int march_mc(int k)
 {
-    static int grid_for_k[17] = {0};
-    if (grid_for_k[K] == 0) {
+    static int forced = -1;
+    if (forced < 0) {
+        const char* e = getenv("ASV_MARCH_MC");
+        const int v = e ? atoi(e) : 0;
+        forced = (v == 31 || v == 63) ? v : 0;
+    }
+    if (forced) return forced;
+    return k <= 3 ? 31 : 63;
+}
+
+using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
+
+template <int K>
+static MarchFn march_fn(int mc)
+{
+    return mc == 31 ? k_jfa_march<K, 31> : k_jfa_march<K, 63>;
+}
+
+static MarchFn march_kernel(int k, int mc)
+{
+    switch (k) {
+    case 1: return march_fn<1>(mc);   case 2: return march_fn<2>(mc);   case 3: return march_fn<3>(mc);
+    case 4: return march_fn<4>(mc);   case 5: return march_fn<5>(mc);   case 6: return march_fn<6>(mc);
+    case 7: return march_fn<7>(mc);   case 8: return march_fn<8>(mc);   case 9: return march_fn<9>(mc);
+    case 10: return march_fn<10>(mc); case 11: return march_fn<11>(mc); case 12: return march_fn<12>(mc);
+    case 13: return march_fn<13>(mc); case 14: return march_fn<14>(mc); case 15: return march_fn<15>(mc);
+    case 16: return march_fn<16>(mc); default: return nullptr;
+    }
+}
+
+// Dynamic LDS of the marching pass: the byte-per-id table for ids < amax,
+// in whole KiB.
+static int march_dyn_bytes(int amax)
+{
+    const int bytes = ((amax > 0 ? amax : 1) + 3) / 4 * 4;
+    return (bytes + 1023) / 1024 * 1024;
+}
+
+// Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
+// 8), cached per step, capacity and table size.
+int march_max_grid(int k, int dyn)
+{
+    static int cache[17][2][kMarchMaxIds / 1024 + 1];
+    if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchMaxIds) return 0;
+    const int mc = march_mc(k);
+    int& slot = cache[k][mc == 63][dyn / 1024];
+    int g = __atomic_load_n(&slot, __ATOMIC_RELAXED);
+    if (g == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_jfa_march<K>, 256, 0);
-        int g = cus * (per_cu > 0 ? per_cu : 1);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc), 256,
+                                                           (size_t)dyn);
+        g = cus * (per_cu > 0 ? per_cu : 1);
         g = (g / 8) * 8;
-        grid_for_k[K] = g < 8 ? 8 : g;
+        if (g < 8) g = 8;
+        __atomic_store_n(&slot, g, __ATOMIC_RELAXED);
     }
-    return grid_for_k[K];
+    return g;
 }
 
-// Segment lengths for n clips of one launch: about per_block work items
-// per resident block overall, each clip's columns x residues cut into equal
-// segments of its ceil(D / K) slices.  grid_limit (> 0) caps the resident
-// blocks.  Fills seg, nseg and item_begin; returns the grid size.
-template <int K>
-static int march_shape(JfaClip* c, int n, int per_block = 4, int grid_limit = 0, int seg_cap = 0)
+// Plans one marching launch over n clips: the segments per column (want,
+// the same for every clip as far as its slices allow; seg_cap > 0 caps the
+// slices per segment) and each clip's first item, the launch's item count
+// and its dynamic LDS bytes.  The grid is the fewest blocks per XCD that
+// still take each XCD's items in the fewest rounds of the resident blocks
+// (grid_limit > 0 caps them).  per_block > 0 fixes want at about per_block
+// items per resident block; per_block = 0 searches want in 1..32 for the
+// least predicted time, rounds x (slices per item + 2), the 2 standing for
+// the three-slab rebuild that starts every item.  Returns the grid size, 0
+// when the marching pass does not apply.
+static int64_t plan_items(int k, JfaClip* c, int n, int64_t want, int seg_cap, int64_t* slices)
 {
-    int grid = march_grid<K>();
-    if (grid_limit > 0 && grid_limit < grid) grid = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
-    int64_t columns = 0;
-    for (int i = 0; i < n; ++i) columns += (int64_t)c[i].tiles_x * c[i].tiles_y * K;
-    const int64_t target = (int64_t)per_block * grid;
-    const int64_t want = columns >= target ? 1 : (target + columns - 1) / columns;
-    int64_t items = 0;
+    int64_t items = 0, sl = 0;
     for (int i = 0; i < n; ++i) {
-        const int steps = (c[i].D + K - 1) / K;
+        const int steps = (c[i].D + k - 1) / k;
         int nseg = (int)(want < steps ? want : steps);
         if (nseg < 1) nseg = 1;
         if (seg_cap > 0 && (steps + nseg - 1) / nseg > seg_cap) nseg = (steps + seg_cap - 1) / seg_cap;
         c[i].seg = (steps + nseg - 1) / nseg;
         c[i].nseg = (steps + c[i].seg - 1) / c[i].seg;
-        c[i].item_begin = (int)items;
-        items += (int64_t)c[i].tiles_x * c[i].tiles_y * K * c[i].nseg;
+        c[i].item_begin = (int)(items < 0x7fffffff ? items : 0x7fffffff);
+        items += (int64_t)c[i].tiles_x * c[i].tiles_y * k * c[i].nseg;
+        sl += (int64_t)c[i].tiles_x * c[i].tiles_y * c[i].D;
     }
-    const int64_t need = ((items + 7) / 8) * 8;
-    return (int)(grid < need ? grid : need);
+    if (slices) *slices = sl;
+    return items;
+}
+
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn)
+{
+    if (k < 1 || k > 16 || n < 1) return 0;
+    int amax = 1;
+    for (int i = 0; i < n; ++i) amax = c[i].A > amax ? c[i].A : amax;
+    if (amax > kMarchMaxIds) return 0;
+    const int lds = march_dyn_bytes(amax);
+    int gmax = march_max_grid(k, lds);
+    if (grid_limit > 0 && grid_limit < gmax) gmax = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
+    const int64_t pmax = gmax / 8;                       // resident blocks per XCD
+    int64_t columns = 0;
+    for (int i = 0; i < n; ++i) columns += (int64_t)c[i].tiles_x * c[i].tiles_y * k;
+    int64_t want = 1;
+    if (per_block > 0) {
+        const int64_t target = (int64_t)per_block * gmax;
+        want = columns >= target ? 1 : (target + columns - 1) / columns;
+    } else {
+        double best = 1e300;
+        for (int64_t w = 1; w <= 32; ++w) {
+            int64_t slices = 0;
+            const int64_t items = plan_items(k, c, n, w, seg_cap, &slices);
+            const int64_t per = (items + 7) / 8;
+            const int64_t rounds = (per + pmax - 1) / pmax;
+            const double t = (double)rounds * ((double)slices / (double)items + 2.0);
+            if (t < best * 0.999) {
+                best = t;
+                want = w;
+            }
+        }
+    }
+    const int64_t items = plan_items(k, c, n, want, seg_cap, nullptr);
+    if (items >= (int64_t)0x7fffffff) return 0;
+    const int64_t per = (items + 7) / 8;
+    const int64_t rounds = (per + pmax - 1) / pmax;
+    int64_t p = (per + rounds - 1) / rounds;
+    if (p < 1) p = 1;
+    if (nitems) *nitems = (int)items;
+    if (dyn) *dyn = lds;
+    return (int)(8 * p);
+}
+
+hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
+                        int dyn)
+{
+    MarchFn fn = march_kernel(k, march_mc(k));
+    if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
+    JfaClip a0 = one;
+    const JfaClip* a1 = dev;
+    int a2 = n, a3 = nitems, a4 = dyn / 4;
+    void* args[] = {(void*)&a0, (void*)&a1, (void*)&a2, (void*)&a3, (void*)&a4};
+    return hipLaunchKernel((const void*)fn, dim3((unsigned)grid), dim3(256), args, (size_t)dyn, st);
+}
+
+hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn)
+{
+    JfaClip none{};
+    return launch_march(st, k, none, dev, n, grid, nitems, dyn);
 }
 
 template <int K>
-static void launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                        const SeedRec* rec, float ratio)
+static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                              const SeedRec* rec, float ratio)
 {
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
-    int nt = tx * ty * d.D;
-    if (d.A <= kDirectIds) {
+    if (d.A <= kMarchMaxIds) {
         JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio};
-        const int g = march_shape<K>(&c, 1);
-        const int nitems = c.tiles_x * c.tiles_y * K * c.nseg;
-        hipLaunchKernelGGL(k_jfa_march<K>, dim3((unsigned)g), dim3(256), 0, st, c, (const JfaClip*)nullptr, 1,
-                           nitems);
-        return;
+        int nitems = 0, dyn = 0;
+        const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
+        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn);
     }
+    int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
                        rec, ratio);
+    return hipGetLastError();
 }
 
-int march_max_grid(int k)
+hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule)
 {
-    switch (k) {
-    case 1: return march_grid<1>();   case 2: return march_grid<2>();   case 3: return march_grid<3>();
-    case 4: return march_grid<4>();   case 5: return march_grid<5>();   case 6: return march_grid<6>();
-    case 7: return march_grid<7>();   case 8: return march_grid<8>();   case 9: return march_grid<9>();
-    case 10: return march_grid<10>(); case 11: return march_grid<11>(); case 12: return march_grid<12>();
-    case 13: return march_grid<13>(); case 14: return march_grid<14>(); case 15: return march_grid<15>();
-    case 16: return march_grid<16>(); default: return 0;
-    }
-}
-
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems)
-{
-    int g = 0;
-    switch (k) {
-    case 1: g = march_shape<1>(c, n, per_block, grid_limit, seg_cap); break;
-    case 2: g = march_shape<2>(c, n, per_block, grid_limit, seg_cap); break;
-    case 3: g = march_shape<3>(c, n, per_block, grid_limit, seg_cap); break;
-    case 4: g = march_shape<4>(c, n, per_block, grid_limit, seg_cap); break;
-    case 5: g = march_shape<5>(c, n, per_block, grid_limit, seg_cap); break;
-    case 6: g = march_shape<6>(c, n, per_block, grid_limit, seg_cap); break;
-    case 7: g = march_shape<7>(c, n, per_block, grid_limit, seg_cap); break;
-    case 8: g = march_shape<8>(c, n, per_block, grid_limit, seg_cap); break;
-    case 9: g = march_shape<9>(c, n, per_block, grid_limit, seg_cap); break;
-    case 10: g = march_shape<10>(c, n, per_block, grid_limit, seg_cap); break;
-    case 11: g = march_shape<11>(c, n, per_block, grid_limit, seg_cap); break;
-    case 12: g = march_shape<12>(c, n, per_block, grid_limit, seg_cap); break;
-    case 13: g = march_shape<13>(c, n, per_block, grid_limit, seg_cap); break;
-    case 14: g = march_shape<14>(c, n, per_block, grid_limit, seg_cap); break;
-    case 15: g = march_shape<15>(c, n, per_block, grid_limit, seg_cap); break;
-    case 16: g = march_shape<16>(c, n, per_block, grid_limit, seg_cap); break;
-    default: return 0;
-    }
-    int64_t items = 0;
-    for (int i = 0; i < n; ++i) items += (int64_t)c[i].tiles_x * c[i].tiles_y * k * c[i].nseg;
-    if (nitems) *nitems = (int)items;
-    return g;
-}
-
-hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems)
-{
-    JfaClip none{};
-    const dim3 g((unsigned)grid), b(256);
-    switch (k) {
-    case 1: hipLaunchKernelGGL(k_jfa_march<1>, g, b, 0, st, none, dev, n, nitems); break;
-    case 2: hipLaunchKernelGGL(k_jfa_march<2>, g, b, 0, st, none, dev, n, nitems); break;
-    case 3: hipLaunchKernelGGL(k_jfa_march<3>, g, b, 0, st, none, dev, n, nitems); break;
-    case 4: hipLaunchKernelGGL(k_jfa_march<4>, g, b, 0, st, none, dev, n, nitems); break;
-    case 5: hipLaunchKernelGGL(k_jfa_march<5>, g, b, 0, st, none, dev, n, nitems); break;
-    case 6: hipLaunchKernelGGL(k_jfa_march<6>, g, b, 0, st, none, dev, n, nitems); break;
-    case 7: hipLaunchKernelGGL(k_jfa_march<7>, g, b, 0, st, none, dev, n, nitems); break;
-    case 8: hipLaunchKernelGGL(k_jfa_march<8>, g, b, 0, st, none, dev, n, nitems); break;
-    case 9: hipLaunchKernelGGL(k_jfa_march<9>, g, b, 0, st, none, dev, n, nitems); break;
-    case 10: hipLaunchKernelGGL(k_jfa_march<10>, g, b, 0, st, none, dev, n, nitems); break;
-    case 11: hipLaunchKernelGGL(k_jfa_march<11>, g, b, 0, st, none, dev, n, nitems); break;
-    case 12: hipLaunchKernelGGL(k_jfa_march<12>, g, b, 0, st, none, dev, n, nitems); break;
-    case 13: hipLaunchKernelGGL(k_jfa_march<13>, g, b, 0, st, none, dev, n, nitems); break;
-    case 14: hipLaunchKernelGGL(k_jfa_march<14>, g, b, 0, st, none, dev, n, nitems); break;
-    case 15: hipLaunchKernelGGL(k_jfa_march<15>, g, b, 0, st, none, dev, n, nitems); break;
-    case 16: hipLaunchKernelGGL(k_jfa_march<16>, g, b, 0, st, none, dev, n, nitems); break;
-    default: return hipErrorInvalidValue;
-    }
+    hipError_t e = launch_fill(st, out, (int64_t)d.W * d.H * d.D, -1);
+    if (e != hipSuccess || d.A < 1) return e;
+    const int m = k + 1 > 3 ? k + 1 : 3;
+    const int64_t threads = (int64_t)d.A * m * m * m;
+    const int64_t blocks = (threads + 255) / 256;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    if (tie_rule == 0)
+        hipLaunchKernelGGL(k_jfa_seeds<0>, dim3((unsigned)blocks), dim3(256), 0, st, d.W, d.H, d.D, d.A, k, m, pos, in,
+                           out, lab, rec, ratio);
+    else
+        hipLaunchKernelGGL(k_jfa_seeds<1>, dim3((unsigned)blocks), dim3(256), 0, st, d.W, d.H, d.D, d.A, k, m, pos, in,
+                           out, lab, rec, ratio);
     return hipGetLastError();
 }
 
@@ -1443,22 +1613,22 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
     int64_t ntiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile) * d.D;
     if (tie_rule == 0 && ntiles < (1ll << 31)) {
         switch (k) {
-        case 1: launch_tile<1>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 2: launch_tile<2>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 3: launch_tile<3>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 4: launch_tile<4>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 5: launch_tile<5>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 6: launch_tile<6>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 7: launch_tile<7>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 8: launch_tile<8>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 9: launch_tile<9>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 10: launch_tile<10>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 11: launch_tile<11>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 12: launch_tile<12>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 13: launch_tile<13>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 14: launch_tile<14>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 15: launch_tile<15>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
-        case 16: launch_tile<16>(st, d, in, out, lab, rec, ratio); return hipGetLastError();
+        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio);
+        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio);
+        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio);
+        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio);
+        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio);
+        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio);
+        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio);
+        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio);
+        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio);
+        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio);
+        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio);
+        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio);
+        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio);
+        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio);
+        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio);
+        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio);
         default: break;
         }
     }

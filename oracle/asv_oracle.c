@@ -543,21 +543,25 @@ typedef struct {
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-/* Candidate distance, flood.gs:107-121.  The quadratic form and the colour
- * distance are evaluated with explicit fused multiply-adds in this order
- * (the reference leaves contraction to the GL driver; this fixes one):
- *   t1 = fma(dz, 2m2, fma(dy, 2m1, dx*m0)),  t2 = fma(dz, 2m4, dy*m3)
- *   q  = max(0, fma(dx, t1, fma(dy, t2, (dz*dz)*m5)))
+/* Candidate distance, flood.gs:107-121.  The reference evaluates
+ * dot((dx^2, 2dxdy, 2dxdz), m0..m2) + dot((dy^2, 2dydz, dz^2), m3..m5) and
+ * clamps negatives to 0, leaving contraction and rounding to the GL driver;
+ * this fixes one order of fused multiply-adds for the same quadratic form
+ * dx(m0dx + 2m1dy + 2m2dz) + dy(m3dy + 2m4dz) + m5dz^2:
+ *   u1 = dz*2m2,  u2 = dz*2m4,  u3 = (dz*dz)*m5          (rounded products)
+ *   t1 = fma(dx, m0, fma(dy, 2m1, u1)),  t2 = fma(dy, m3, u2)
+ *   q  = fma(dx, t1, fma(dy, t2, u3)), then +0 if its sign bit is set
  *   cd = fma(c2, c2, fma(c1, c1, c0*c0)),    d = fma(ratio, q, cd)
- * which is dx(m0dx+2m1dy+2m2dz) + dy(m3dy+2m4dz) + m5dz^2 = delta^T M delta. */
+ * The u terms depend on the seed and the slice only. */
 static inline float cand_dist(const orc_seed* s, int x, int y, int z, const float lab[3], float ratio)
 {
     float dx = (float)(x - s->x), dy = (float)(y - s->y), dz = (float)(z - s->z);
     float m1d = 2.0f * s->m[1], m2d = 2.0f * s->m[2], m4d = 2.0f * s->m[4];
-    float t1 = fmaf(dz, m2d, fmaf(dy, m1d, dx * s->m[0]));
-    float t2 = fmaf(dz, m4d, dy * s->m[3]);
-    float q = fmaf(dx, t1, fmaf(dy, t2, (dz * dz) * s->m[5]));
-    if (q < 0.0f) q = 0.0f;                                   /* flood.gs:117-120 */
+    float u1 = dz * m2d, u2 = dz * m4d, u3 = (dz * dz) * s->m[5];
+    float t1 = fmaf(dx, s->m[0], fmaf(dy, m1d, u1));
+    float t2 = fmaf(dy, s->m[3], u2);
+    float q = fmaf(dx, t1, fmaf(dy, t2, u3));
+    if (signbit(q)) q = 0.0f;                                 /* flood.gs:117-120 */
     float c0 = lab[0] - s->lab[0], c1 = lab[1] - s->lab[1], c2 = lab[2] - s->lab[2];
     float cd = fmaf(c2, c2, fmaf(c1, c1, c0 * c0));           /* flood.gs:116 */
     return fmaf(ratio, q, cd);                                /* flood.gs:121 */
