@@ -56,6 +56,8 @@ SIGNATURES = {
     "asv_write_render_ppm": (ctypes.c_int, [_vp, _i32, _i32, _i32, _i32, ctypes.c_char_p]),
     "asv_eval_metrics": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "asv_plan_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
+    "asv_connectivity": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "asv_connectivity_labels": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp]),
     "asv_download_labels_mapped": (ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _vp]),
     "asv_stream_block_into": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "asv_stream_checksum": (ctypes.c_int, [_vp, _vp]),

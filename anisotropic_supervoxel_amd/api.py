@@ -253,6 +253,14 @@ class Clip:
     def run(self):
         check(self.lib.asv_run(self.h, self.stream))
 
+    def connectivity(self, supsz=0):
+        """Image::enforce_connectivity_3d on the clip's current labels
+        (asv_connectivity); the clip's labels are rewritten.  Returns the
+        number of segments kept."""
+        n = ctypes.c_int32(0)
+        check(self.lib.asv_connectivity(self.h, int(supsz), ctypes.byref(n), self.stream))
+        return n.value
+
     def run_timed(self):
         ms = np.zeros(6, np.float32)
         n = ctypes.c_int32(0)

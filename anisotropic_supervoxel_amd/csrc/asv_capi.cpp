@@ -782,6 +782,34 @@ int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, in
     return ASV_OK;
 }
 
+int asv_connectivity(asv_ctx* c, int32_t supsz, int32_t* num_labels, void* stream)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->last_iter < 1) return fail(ASV_ERR_STATE, "connectivity before any Lloyd iteration");
+    const int64_t V = c->V();
+    const int sz = supsz > 0 ? supsz : (int)(V / (c->p.seeds_num > 0 ? c->p.seeds_num : 1));   // Image.cpp:1833
+    int32_t n = 0;
+    hipError_t e = connect_labels(S(stream), c->labels[c->cur], c->labels[1 - c->cur], c->d.W, c->d.H, c->d.D, sz, &n);
+    if (e != hipSuccess) return fail(ASV_ERR_HIP, "connectivity: %s", hipGetErrorString(e));
+    c->cur = 1 - c->cur;
+    if (num_labels) *num_labels = n;
+    return ASV_OK;
+}
+
+int asv_connectivity_labels(const int32_t* in, int32_t* out, int32_t width, int32_t height, int32_t depth,
+                            int32_t supsz, int32_t* num_labels, void* stream)
+{
+    if (!in || !out || in == out || width < 1 || height < 1 || depth < 1 || supsz < 0)
+        return fail(ASV_ERR_ARG, "bad connectivity arguments");
+    if ((int64_t)width * height * depth >= 0x7fffffffLL) return fail(ASV_ERR_ARG, "volume too large for int32 indices");
+    int32_t n = 0;
+    hipError_t e = connect_labels(S(stream), in, out, width, height, depth, supsz, &n);
+    if (e != hipSuccess) return fail(ASV_ERR_HIP, "connectivity: %s", hipGetErrorString(e));
+    if (num_labels) *num_labels = n;
+    return ASV_OK;
+}
+
 int asv_host_alloc(int64_t bytes, void** out)
 {
     if (!out || bytes <= 0) return fail(ASV_ERR_ARG, "bad host allocation request");

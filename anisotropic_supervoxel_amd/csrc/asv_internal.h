@@ -97,4 +97,12 @@ int64_t post_scratch_words(int64_t V);
 hipError_t launch_postprocess(hipStream_t st, const int32_t* in, int32_t* out, int64_t V, int mode,
                               const int32_t* table, int n_table, long long* scratch);
 
+// 3D connectivity enforcement (GPU-AS/Image.cpp:1938-2062) from V = W*H*D
+// labels `in` into `out` (device, in != out): components of the same-label
+// 24-neighbourhood smaller than supsz / 4 voxels take the adjacent label,
+// the others are numbered from 0 in raster order of their first voxel.
+// Allocates its own scratch, synchronises st; *num_labels = the count.
+hipError_t connect_labels(hipStream_t st, const int32_t* in, int32_t* out, int W, int H, int D, int supsz,
+                          int32_t* num_labels);
+
 }  // namespace asv

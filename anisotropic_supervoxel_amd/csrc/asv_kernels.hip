@@ -899,7 +899,9 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     constexpr int RPI = 64 / R;               // staged rows per wave instruction
     constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
     constexpr MaskT kBadBit = (MaskT)1 << MC;
-    constexpr bool kLabAhead = K >= 3;        // LAB of the next slice loaded during this one
+    // LAB of the next slice loaded during this one; at K = 2 the extra
+    // register costs a block per CU, more than the latency it hides
+    constexpr bool kLabAhead = K != 2;
     extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
     __shared__ uint8_t s_idx[3 * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)

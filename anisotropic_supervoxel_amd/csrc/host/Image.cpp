@@ -82,6 +82,16 @@ bool Image::check_dead_seeds(int* labels)
     return true;
 }
 
+bool Image::enforce_connectivity_3d(int* labels, int& label_num)
+{
+    int32_t n = 0;
+    if (!ok(asv_connectivity(ctx, 0, &n, nullptr), "enforce_connectivity_3d")) return false;   // SUPSZ = V / K
+    if (!ok(asv_download_labels(ctx, labels, nullptr), "enforce_connectivity_3d labels")) return false;
+    final_label_num = n;
+    label_num = n;
+    return true;
+}
+
 bool Image::print_sv_txt(int* labels, std::string dirName, bool output_label, bool connected)
 {
     if (!output_label) return true;

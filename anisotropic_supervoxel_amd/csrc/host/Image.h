@@ -34,6 +34,10 @@ public:
     bool Ld_next_seeds_matrix(int Lloyd_curr_iter);
     // GPU-AS/Image.cpp:1579-1720: labels is caller-allocated int[W*H*D].
     bool check_dead_seeds(int* labels);
+    // GPU-AS/Image.cpp:1808-1837 with EnforceLabelConnectivity3D
+    // (:1938-2062), on the raw labels of the last pass: labels is
+    // caller-allocated int[W*H*D]; label_num receives the segments kept.
+    bool enforce_connectivity_3d(int* labels, int& label_num);
     // GPU-AS/Image.cpp:1780-1806 and :1883-1935.
     bool print_sv_txt(int* labels, std::string dirName, bool output_label, bool connected);
     bool print_render(int* labels, std::string prefix);

@@ -45,7 +45,7 @@ void mkdirs(const std::string& path)
 
 Render::Render()
     : Lloyd_ends(false), Lloyd_curr_iter(0), Lloyd_max_iter(20), width(0), height(0), depth(0),
-      output_label(true), output_render(true), ms_total(0), ms_loop(0)
+      output_label(true), output_render(true), connect(false), ms_total(0), ms_loop(0)
 {
 }
 
@@ -119,12 +119,19 @@ int Render::run_arrays(const uint8_t* rgb, const float* fwd, const float* bwd, i
         if (!image.centroids_next_seeds(Lloyd_curr_iter)) return -1;
     }
     std::vector<int> labels((size_t)W * H * D);
-    if (!image.check_dead_seeds(labels.data())) return -1;       // synchronises the stream
+    if (connect) {                                                // GPU-AS/Render.cpp:324-341
+        int label_num = 0;
+        if (!image.enforce_connectivity_3d(labels.data(), label_num)) return -1;
+        std::printf("label_num=%d\n", label_num);
+    } else if (!image.check_dead_seeds(labels.data())) {          // synchronises the stream
+        return -1;
+    }
     ms_loop = now_ms() - tl;
-    std::printf("Lloyd loop + check_dead_seeds, time_ms:%.2f\n", ms_loop);
+    std::printf(connect ? "Lloyd loop + connectivity, time_ms:%.2f\n" : "Lloyd loop + check_dead_seeds, time_ms:%.2f\n",
+                ms_loop);
     if (!out_prefix.empty()) {
         double ts = now_ms();
-        if (!image.print_sv_txt(labels.data(), out_prefix, output_label, false)) return -1;
+        if (!image.print_sv_txt(labels.data(), out_prefix, output_label, connect)) return -1;
         if (output_render) {
             std::string rdir = out_prefix + "_render/";
             mkdirs(rdir);

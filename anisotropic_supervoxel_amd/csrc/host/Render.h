@@ -34,5 +34,6 @@ public:
     int width, height, depth;
     bool output_label;       // write the label text (the reference hard-codes false)
     bool output_render;      // write the pseudo-colour frames
+    bool connect;            // 3D connectivity in place of check_dead_seeds (the reference's step 6.2)
     double ms_total, ms_loop;
 };

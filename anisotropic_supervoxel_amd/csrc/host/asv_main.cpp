@@ -4,7 +4,8 @@
 //   asv --synth SEED,W,H,D,ELLIPSES,VMAX,SIGMA [--write-clip DIR]
 //   asv --dataset DIR --video NAME
 //   common: --K 500 --iters 20 --compactness 8 --save DIR --db NAME
-//           --no-txt --no-render --eval
+//           --no-txt --no-render --eval --connect (3D connectivity instead of
+//           check_dead_seeds)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,7 +19,7 @@ static void usage()
 {
     std::printf("usage: asv (--synth SEED,W,H,D,E,VMAX,SIGMA | --dataset DIR --video NAME) [--K n] [--iters n]\n"
                 "           [--compactness c] [--save DIR] [--db NAME] [--write-clip DIR] [--no-txt] [--no-render]\n"
-                "           [--eval]\n");
+                "           [--eval] [--connect]\n");
 }
 
 int main(int argc, char** argv)
@@ -26,7 +27,7 @@ int main(int argc, char** argv)
     std::string synth, dataset, video, save = "results/", db = "synthetic", write_clip;
     int K = 500, iters = 20;
     float compactness = 8.0f;
-    bool txt = true, render = true, eval = false;
+    bool txt = true, render = true, eval = false, connect = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -45,6 +46,7 @@ int main(int argc, char** argv)
         else if (a == "--no-txt") txt = false;
         else if (a == "--no-render") render = false;
         else if (a == "--eval") eval = true;
+        else if (a == "--connect") connect = true;
         else { usage(); return 2; }
     }
     if (!save.empty() && save.back() != '/') save += "/";
@@ -52,6 +54,7 @@ int main(int argc, char** argv)
     r.Lloyd_max_iter = iters;
     r.output_label = txt;
     r.output_render = render;
+    r.connect = connect;
     if (!synth.empty()) {
         unsigned long long seed;
         int W, H, D, E;

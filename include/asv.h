@@ -173,6 +173,24 @@ int asv_postprocess_device(asv_ctx* ctx, const int64_t* sums, int32_t* dead_out,
 int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
                            int32_t* dead_out, int32_t* final_label_num, void* stream);
 
+/* 3D connectivity enforcement (Image::enforce_connectivity_3d,
+ * GPU-AS/Image.cpp:1808-1837, over EnforceLabelConnectivity3D :1938-2062)
+ * on the context's current labels (after a run: the raw labels of the last
+ * JFA pass, which is what the reference reads; after
+ * asv_postprocess_device: the post-processed ones).  Segments are the
+ * connected components of equal labels under the reference's 24-voxel
+ * neighbourhood; one of fewer than supsz / 4 voxels takes the adjacent
+ * label, the others are numbered from 0 in raster order.  supsz <= 0 uses
+ * the reference's V / K (K the requested supervoxel count).  The labels of
+ * the context are rewritten (later downloads read the result);
+ * *num_labels receives the number of segments kept.  Synchronises stream. */
+int asv_connectivity(asv_ctx* ctx, int32_t supsz, int32_t* num_labels, void* stream);
+
+/* The same between two device buffers of W x H x D labels (in != out), GL
+ * raster order, for a label volume produced elsewhere.  Synchronises. */
+int asv_connectivity_labels(const int32_t* in, int32_t* out, int32_t width, int32_t height, int32_t depth,
+                            int32_t supsz, int32_t* num_labels, void* stream);
+
 /* Page-locked host memory for frame and label buffers (faster transfers).
  * No counterpart in the reference; for callers that stage frames. */
 int asv_host_alloc(int64_t bytes, void** out);

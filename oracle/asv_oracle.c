@@ -799,3 +799,68 @@ ORC_API int orc_run(const orc_params* P, const uint8_t* rgb, const float* fwd, c
 }
 
 ORC_API int orc_seed_record_floats(void) { return (int)(sizeof(orc_seed) / sizeof(float)); }
+
+/* 3D connectivity enforcement, GPU-AS/Image.cpp:1938-2062
+ * (EnforceLabelConnectivity3D).  Voxels are visited in raster order (x
+ * fastest, then y, then z); an unvisited voxel starts a new segment, grown
+ * breadth-first over the 24 neighbours (dz in -1..1 times the 8 in-plane
+ * offsets; the pure z neighbours (0, 0, +-1) are not among them) that carry
+ * the same input label.  Before growing, the segment's first voxel scans
+ * those neighbours (dz -1, 0, 1; in-plane order (-1,0), (-1,-1), (0,-1),
+ * (1,-1), (1,0), (1,1), (0,1), (-1,1)) and keeps the output label of the
+ * last one already visited as the adjacent label; the variable persists
+ * from one segment to the next and starts at 0.  A segment of fewer than
+ * supsz / 4 voxels takes the adjacent label and gives its number back;
+ * every other segment gets the next number from 0.  Returns the count of
+ * numbers given. */
+ORC_API int orc_connectivity(const int32_t* labels, int W, int H, int D, int supsz, int32_t* out)
+{
+    static const int ox[8] = {-1, -1, 0, 1, 1, 1, 0, -1};
+    static const int oy[8] = {0, -1, -1, -1, 0, 1, 1, 1};
+    const size_t V = (size_t)W * H * D;
+    int* queue = (int*)malloc(V * sizeof(int));
+    if (!queue) return -1;
+    for (size_t i = 0; i < V; ++i) out[i] = -1;
+    int next = 0, adj = 0;
+    for (size_t v0 = 0; v0 < V; ++v0) {
+        if (out[v0] >= 0) continue;
+        const int x0 = (int)(v0 % (size_t)W), y0 = (int)((v0 / (size_t)W) % (size_t)H), z0 = (int)(v0 / ((size_t)W * H));
+        out[v0] = next;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int z = z0 + dz;
+            if (z < 0 || z >= D) continue;
+            for (int n = 0; n < 8; ++n) {
+                const int x = x0 + ox[n], y = y0 + oy[n];
+                if (x < 0 || x >= W || y < 0 || y >= H) continue;
+                const int32_t l = out[((size_t)z * H + y) * W + x];
+                if (l >= 0) adj = l;
+            }
+        }
+        size_t count = 1;
+        queue[0] = (int)v0;
+        for (size_t c = 0; c < count; ++c) {
+            const size_t q = (size_t)queue[c];
+            const int qx = (int)(q % (size_t)W), qy = (int)((q / (size_t)W) % (size_t)H), qz = (int)(q / ((size_t)W * H));
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int z = qz + dz;
+                if (z < 0 || z >= D) continue;
+                for (int n = 0; n < 8; ++n) {
+                    const int x = qx + ox[n], y = qy + oy[n];
+                    if (x < 0 || x >= W || y < 0 || y >= H) continue;
+                    const size_t u = ((size_t)z * H + y) * W + x;
+                    if (out[u] < 0 && labels[u] == labels[v0]) {
+                        out[u] = next;
+                        queue[count++] = (int)u;
+                    }
+                }
+            }
+        }
+        if ((long long)count < (long long)(supsz / 4)) {
+            for (size_t c = 0; c < count; ++c) out[queue[c]] = adj;
+        } else {
+            ++next;
+        }
+    }
+    free(queue);
+    return next;
+}

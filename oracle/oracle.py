@@ -59,6 +59,7 @@ def load():
             "orc_finalize": (None, [vp, vp, i, i, i, i, vp]),
             "orc_run": (i, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_seed_record_floats": (i, []),
+            "orc_connectivity": (i, [vp, i, i, i, i, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -208,3 +209,14 @@ def run(rgb, fwd, bwd, K, iters, compactness=8.0, tie_rule=0, angle_gate=1, sche
     if r < 0:
         raise ValueError(f"orc_run failed: {r}")
     return labels, sums, trace, A
+
+
+def connectivity(labels, supsz):
+    """GPU-AS/Image.cpp:1938-2062 restated: (connected labels [D,H,W], count)."""
+    labels = np.ascontiguousarray(labels, np.int32)
+    D, H, W = labels.shape
+    out = np.zeros_like(labels)
+    n = load().orc_connectivity(_p(labels), W, H, D, int(supsz), _p(out))
+    if n < 0:
+        raise MemoryError("orc_connectivity")
+    return out, n
