@@ -57,6 +57,7 @@ SIGNATURES = {
     "asv_eval_metrics": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "asv_plan_seeds": (ctypes.c_int, [_vp, _vp, _i32]),
     "asv_connectivity": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "asv_initialization": (ctypes.c_int, [_vp, _i32]),
     "asv_connectivity_labels": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp]),
     "asv_download_labels_mapped": (ctypes.c_int, [_vp, _vp, _i32, _i32, _vp, _i64, _vp, _vp]),
     "asv_stream_block_into": (ctypes.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -105,7 +106,7 @@ class Params(ctypes.Structure):
         ("width", _i32), ("height", _i32), ("depth", _i32),
         ("seeds_num", _i32), ("lloyd_iters", _i32), ("compactness", _f32),
         ("tie_rule", _i32), ("angle_gate", _i32), ("schedule", _i32),
-        ("frames_prepared", _i32),
+        ("frames_prepared", _i32), ("seed_init", _i32),
     ]
 
 

@@ -25,7 +25,7 @@ from ._lib import AsvError, Params, check, ptr  # noqa: F401
 
 def default_params(width, height, depth, seeds_num, **kw):
     """asv_default_params, then keyword overrides (lloyd_iters, compactness,
-    tie_rule, angle_gate, schedule, frames_prepared)."""
+    tie_rule, angle_gate, schedule, frames_prepared, seed_init)."""
     lib = _lib.load()
     p = Params()
     lib.asv_default_params(ctypes.byref(p), width, height, depth, seeds_num)
@@ -252,6 +252,11 @@ class Clip:
 
     def run(self):
         check(self.lib.asv_run(self.h, self.stream))
+
+    def initialization(self, density=True):
+        """Image::initialization (asv_initialization): density-based seed
+        re-placement at the next prepare (or back to the plain grid)."""
+        check(self.lib.asv_initialization(self.h, 1 if density else 0))
 
     def connectivity(self, supsz=0):
         """Image::enforce_connectivity_3d on the clip's current labels

@@ -35,7 +35,7 @@ int fail(int code, const char* fmt, const char* detail = nullptr)
 
 // Grid seeding, GPU-AS/Image.cpp:20-92 (integer divisions, double pow,
 // float sqrt and truncations as written there).  Returns A or -1.
-int grid_seeds(int W, int H, int D, int K, std::vector<int>* xyz, int* s_out)
+int grid_seeds(int W, int H, int D, int K, std::vector<int>* xyz, int* s_out, int* dims = nullptr)
 {
     int V = W * H * D;
     double voxel_size = (double)(V / K);
@@ -71,6 +71,7 @@ int grid_seeds(int W, int H, int D, int K, std::vector<int>* xyz, int* s_out)
         if (td < d_rem) ++td;
     }
     if (s_out) *s_out = s;
+    if (dims) { dims[0] = dNum; dims[1] = hNum; dims[2] = wNum; }
     return count;
 }
 
@@ -136,6 +137,7 @@ bool valid_params(const asv_params* p)
     if (!(p->compactness > 0.0f)) return false;
     if (p->tie_rule != ASV_TIE_LOWEST_ID && p->tie_rule != ASV_TIE_SCAN_ORDER) return false;
     if (p->schedule != ASV_SCHED_REFERENCE && p->schedule != ASV_SCHED_HALVING_PLUS_ONE) return false;
+    if (p->seed_init != ASV_INIT_GRID && p->seed_init != ASV_INIT_DENSITY) return false;
     return true;
 }
 
@@ -296,6 +298,87 @@ int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out
     return ASV_OK;
 }
 
+// Density-based re-placement of the reference grid (Image::initialization,
+// GPU-AS/Image.cpp:350-620): the device sums each grid seed's colour-gradient
+// density, the host walks the density prefix sums (float, as the reference's
+// host code) and re-places the A seeds; the result replaces the context's
+// initial seeds.  Always starts from the reference grid, so a second
+// prepare of the same context gives the same seeds.
+int density_reseed(asv_ctx* c, hipStream_t st)
+{
+    const int W = c->d.W, H = c->d.H, D = c->d.D, A = c->d.A;
+    const int dN = c->gdims[0], hN = c->gdims[1], wN = c->gdims[2];
+    const std::vector<int>& g = c->grid_ref;
+    if (dN * hN * wN != A || (int)g.size() != 3 * A) return fail(ASV_ERR_STATE, "density init needs the reference grid");
+    // box of grid index i along an axis of n seeds (Image.cpp:360-433)
+    auto box = [](const std::vector<int>& p, int n, int N, int i, int* lo, int* hi) {
+        *lo = i > 0 ? (int)std::floor(0.5 * (((double)p[i - 1] + 0.5) + ((double)p[i] + 0.5))) : 0;
+        *hi = i < n - 1 ? (int)std::floor(0.5 * (((double)p[i + 1] + 0.5) + ((double)p[i] + 0.5))) : N - 1;
+    };
+    std::vector<int> px((size_t)wN), py((size_t)hN), pz((size_t)dN);
+    for (int i = 0; i < wN; ++i) px[i] = g[3 * i];
+    for (int j = 0; j < hN; ++j) py[j] = g[3 * (j * wN) + 1];
+    for (int t = 0; t < dN; ++t) pz[t] = g[3 * (t * wN * hN) + 2];
+    // grid index of every coordinate; a later box overwrites (Image.cpp:436-458)
+    std::vector<int> maps((size_t)(W + H + D), -1);
+    int* mx = maps.data();
+    int* my = mx + W;
+    int* mz = my + H;
+    int lo, hi;
+    for (int i = 0; i < wN; ++i) { box(px, wN, W, i, &lo, &hi); for (int x = std::max(lo, 0); x <= std::min(hi, W - 1); ++x) mx[x] = i; }
+    for (int j = 0; j < hN; ++j) { box(py, hN, H, j, &lo, &hi); for (int y = std::max(lo, 0); y <= std::min(hi, H - 1); ++y) my[y] = j; }
+    for (int t = 0; t < dN; ++t) { box(pz, dN, D, t, &lo, &hi); for (int z = std::max(lo, 0); z <= std::min(hi, D - 1); ++z) mz[z] = t; }
+    const int nslot = (int)(float)(_Float16)(float)(A - 1) + 1;     // ids pass through R16F
+    int* d_maps = nullptr;
+    unsigned long long* d_acc = nullptr;
+    std::vector<unsigned long long> acc((size_t)nslot);
+    hipError_t e = hipMalloc((void**)&d_maps, sizeof(int) * maps.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&d_acc, sizeof(unsigned long long) * (size_t)nslot);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_maps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_zero_u64(st, reinterpret_cast<uint64_t*>(d_acc), nslot);
+    if (e == hipSuccess)
+        e = launch_density(st, c->d, c->rgb, c->p.frames_prepared, d_maps, d_maps + W, d_maps + W + H, wN, hN, nslot,
+                           d_acc);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(acc.data(), d_acc, sizeof(unsigned long long) * (size_t)nslot, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_maps) (void)hipFree(d_maps);
+    if (d_acc) (void)hipFree(d_acc);
+    if (e != hipSuccess) return fail(ASV_ERR_HIP, "density init: %s", hipGetErrorString(e));
+    // densities, prefix sums and the walk (Image.cpp:545-616), in float
+    std::vector<float> len((size_t)A);
+    for (int i = 0; i < A; ++i) {
+        const float res = (float)(3.0 + (double)acc[(size_t)i] / 4096.0);   // clear colour 3 + the blended sum
+        const float d = (float)(0.001 * (double)res);
+        len[(size_t)i] = i == 0 ? d : d + len[(size_t)i - 1];
+    }
+    const float interval = len[(size_t)A - 1] / (float)(A + 1);
+    std::vector<int> xyz((size_t)A * 3);
+    int j1 = 0, placed = 0;
+    for (int i = 0; i < A;) {
+        const float seed_in = (float)(i + 1) * interval;
+        if (seed_in < len[(size_t)j1]) {
+            box(px, wN, W, j1 % wN, &lo, &hi);
+            const float frac = j1 == 0 ? seed_in / len[0]
+                                       : (seed_in - len[(size_t)j1 - 1]) / (len[(size_t)j1] - len[(size_t)j1 - 1]);
+            xyz[3 * (size_t)i] = (int)std::floor((float)lo + (float)(hi - lo) * frac);
+            xyz[3 * (size_t)i + 1] = g[3 * (size_t)j1 + 1];
+            xyz[3 * (size_t)i + 2] = g[3 * (size_t)j1 + 2];
+            ++i;
+            ++placed;
+        } else if (++j1 == A) {
+            break;
+        }
+    }
+    if (placed != A) return fail(ASV_ERR_STATE, "density init placed fewer seeds than the grid");
+    c->grid_host = xyz;
+    c->grid_staging.resize((size_t)A);
+    for (int i = 0; i < A; ++i)
+        c->grid_staging[(size_t)i] = make_int4(xyz[3 * (size_t)i], xyz[3 * (size_t)i + 1], xyz[3 * (size_t)i + 2], 0);
+    HIP_TRY(hipMemcpyAsync(c->grid, c->grid_staging.data(), sizeof(int4) * (size_t)A, hipMemcpyHostToDevice, st));
+    return ASV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -305,11 +388,17 @@ int asv_create(const asv_params* p, asv_ctx** out)
     if (!out) return fail(ASV_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!valid_params(p)) return fail(ASV_ERR_ARG, "invalid parameters");
-    int s = 0;
+    int s = 0, dims[3] = {0, 0, 0};
     std::vector<int> xyz;
-    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &xyz, &s);
+    int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, &xyz, &s, dims);
     if (A <= 0) return fail(ASV_ERR_ARG, "grid seeding places no seed for these sizes");
-    return create_ctx(p, std::move(xyz), s, out);
+    std::vector<int> ref = xyz;
+    int r = create_ctx(p, std::move(xyz), s, out);
+    if (r == ASV_OK) {
+        (*out)->grid_ref = std::move(ref);
+        for (int i = 0; i < 3; ++i) (*out)->gdims[i] = dims[i];
+    }
+    return r;
 }
 
 int asv_create_seeded(const asv_params* p, const int32_t* xyz, int32_t n, int32_t grid_step, asv_ctx** out)
@@ -343,7 +432,7 @@ bool asv::ctx_reusable(const asv_ctx* c, const asv_params* p, int32_t seeds)
     return c && p && c->p.width == p->width && c->p.height == p->height && c->depth_cap >= p->depth &&
            c->seed_cap >= seeds && c->p.lloyd_iters == p->lloyd_iters && c->p.compactness == p->compactness &&
            c->p.tie_rule == p->tie_rule && c->p.angle_gate == p->angle_gate && c->p.schedule == p->schedule &&
-           c->p.frames_prepared == p->frames_prepared && !c->trace_on;
+           c->p.frames_prepared == p->frames_prepared && c->p.seed_init == p->seed_init && !c->trace_on;
 }
 
 int32_t asv::ctx_seed_cap(const asv_ctx* c) { return c ? c->seed_cap : 0; }
@@ -408,6 +497,8 @@ int asv_reseed(asv_ctx* c, const int32_t* xyz, int32_t n, int32_t grid_step, int
     c->ratio = (c->p.compactness / (float)grid_step) * (c->p.compactness / (float)grid_step);
     schedule_list(c->p.schedule, grid_step, c->d.W, c->steps, c->mult);
     c->grid_host.assign(xyz, xyz + 3 * (size_t)n);
+    c->grid_ref.clear();                       // no longer the reference grid
+    c->gdims[0] = c->gdims[1] = c->gdims[2] = 0;
     c->grid_staging.resize((size_t)n);
     for (int i = 0; i < n; ++i) c->grid_staging[(size_t)i] = make_int4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0);
     HIP_TRY(hipMemcpyAsync(c->grid, c->grid_staging.data(), sizeof(int4) * (size_t)n, hipMemcpyHostToDevice, S(stream)));
@@ -466,6 +557,7 @@ int asv_prepare(asv_ctx* c, void* stream)
     HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->fwd, c->bwd, c->lut, c->lab, c->field,
                            c->fix_list, c->fix_count));
     HIP_TRY(launch_fixup(st, c->d, c->fwd, c->bwd, c->field, c->fix_list, c->fix_count));
+    if (c->p.seed_init == ASV_INIT_DENSITY && c->gdims[0] > 0 && (r = density_reseed(c, st))) return r;
     c->prepared = true;
     return ASV_OK;
 }
@@ -779,6 +871,28 @@ int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, in
     (void)hipFree(scratch);
     if (table) (void)hipFree(table);
     if (e != hipSuccess) return fail(ASV_ERR_HIP, "post-process: %s", hipGetErrorString(e));
+    return ASV_OK;
+}
+
+int asv_initialization(asv_ctx* c, int32_t seed_init)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (seed_init != ASV_INIT_GRID && seed_init != ASV_INIT_DENSITY) return fail(ASV_ERR_ARG, "unknown seed placement");
+    if (seed_init == ASV_INIT_DENSITY && c->gdims[0] == 0)
+        return fail(ASV_ERR_STATE, "density placement needs the reference grid (context made from explicit seeds)");
+    if (c->p.seed_init != seed_init) {
+        c->p.seed_init = seed_init;
+        if (seed_init == ASV_INIT_GRID) {               // back to the plain grid
+            c->grid_host = c->grid_ref;
+            const int A = c->d.A;
+            c->grid_staging.resize((size_t)A);
+            for (int i = 0; i < A; ++i)
+                c->grid_staging[(size_t)i] = make_int4(c->grid_ref[3 * (size_t)i], c->grid_ref[3 * (size_t)i + 1],
+                                                       c->grid_ref[3 * (size_t)i + 2], 0);
+            HIP_TRY(hipMemcpy(c->grid, c->grid_staging.data(), sizeof(int4) * (size_t)A, hipMemcpyHostToDevice));
+        }
+        c->prepared = false;
+    }
     return ASV_OK;
 }
 

@@ -15,6 +15,8 @@ struct asv_ctx {
     std::vector<int> steps;
     std::vector<float> mult;
     std::vector<int> grid_host;
+    std::vector<int> grid_ref;      // the reference grid (asv_create), empty for seeded contexts
+    int gdims[3] = {0, 0, 0};       // its dNum, hNum, wNum
     std::vector<int4> grid_staging;
     int64_t bytes = 0;
     bool have_frames = false, have_flow = false, prepared = false;

@@ -44,6 +44,12 @@ hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int
                           uint32_t* lab, float4* field, int* fix_list, int* fix_count);
 hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd,
                         float4* field, const int* fix_list, const int* fix_count);
+// Colour-gradient density of every grid seed's box (GPU-AS/Image.cpp:350-560):
+// acc[slot] += 4096 * g over the voxels whose float16 grid id addresses
+// slot (< nslot); mx / my / mz map x / y / z to the grid index along each
+// axis.
+hipError_t launch_density(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared, const int* mx,
+                          const int* my, const int* mz, int wnum, int hnum, int nslot, unsigned long long* acc);
 hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, const uint32_t* lab,
                             int4* pos, float4* slab);
 hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab,

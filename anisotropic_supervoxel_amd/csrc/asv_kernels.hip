@@ -261,6 +261,36 @@ __device__ __forceinline__ void store_field(float4* field, size_t v, const doubl
     field[2 * v + 1] = make_float4(f[4], f[5], p ? 1.0f : 0.0f, 0.0f);
 }
 
+// (R, G, B) of voxel (x, y, t), GL row order, as the reference's frame
+// texture holds it (Render::cvMat2TexureInput, GPU-AS/Render.cpp:354-359):
+// GaussianBlur 3x3 [1,2,1]^2/16 with reflect-101 borders and round half up,
+// on the image row that becomes GL row y after the vertical flip; frames
+// already prepared are read as they are.
+__device__ __forceinline__ uint3 frame_rgb(const uint8_t* __restrict__ rgb, int prepared, int W, int H, int x, int y,
+                                           int t)
+{
+    if (prepared) {
+        const uint8_t* p = rgb + (((size_t)t * H + y) * W + x) * 3;
+        return make_uint3(p[0], p[1], p[2]);
+    }
+    const uint8_t* fr = rgb + (size_t)t * H * W * 3;
+    const int yi = H - 1 - y;
+    uint32_t ar = 0, ag = 0, ab = 0;
+#pragma unroll
+    for (int j = -1; j <= 1; ++j) {
+        const int yy = reflect101(yi + j, H);
+        const uint32_t wj = j == 0 ? 2u : 1u;
+#pragma unroll
+        for (int i = -1; i <= 1; ++i) {
+            const int xx = reflect101(x + i, W);
+            const uint32_t w = wj * (i == 0 ? 2u : 1u);
+            const uint8_t* p = fr + ((size_t)yy * W + xx) * 3;
+            ar += w * p[0]; ag += w * p[1]; ab += w * p[2];
+        }
+    }
+    return make_uint3((ar + 8u) >> 4, (ag + 8u) >> 4, (ab + 8u) >> 4);
+}
+
 // Kernel (a): blur + flip + LAB + metric for one voxel per thread.  A voxel
 // whose normal equations are rank deficient is queued for k_fixup.
 __global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint8_t* __restrict__ rgb,
@@ -280,30 +310,8 @@ __global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint
     if (x >= W || y >= H) return;
     size_t v = ((size_t)t * H + y) * W + x;
 
-    uint32_t cr, cg, cb;
-    if (prepared) {
-        const uint8_t* p = rgb + v * 3;
-        cr = p[0]; cg = p[1]; cb = p[2];
-    } else {
-        // GaussianBlur 3x3 [1,2,1]^2/16, reflect-101, round half up, on the
-        // image row that becomes GL row y after the vertical flip.
-        const uint8_t* fr = rgb + (size_t)t * H * W * 3;
-        int yi = H - 1 - y;
-        uint32_t ar = 0, ag = 0, ab = 0;
-#pragma unroll
-        for (int j = -1; j <= 1; ++j) {
-            int yy = reflect101(yi + j, H);
-            uint32_t wj = j == 0 ? 2u : 1u;
-#pragma unroll
-            for (int i = -1; i <= 1; ++i) {
-                int xx = reflect101(x + i, W);
-                uint32_t w = wj * (i == 0 ? 2u : 1u);
-                const uint8_t* p = fr + ((size_t)yy * W + xx) * 3;
-                ar += w * p[0]; ag += w * p[1]; ab += w * p[2];
-            }
-        }
-        cr = (ar + 8u) >> 4; cg = (ag + 8u) >> 4; cb = (ab + 8u) >> 4;
-    }
+    const uint3 c3 = frame_rgb(rgb, prepared, W, H, x, y, t);
+    const uint32_t cr = c3.x, cg = c3.y, cb = c3.z;
     lab[v] = lab_pack(cr, cg, cb, lin);
 
     double N[21], c[6], m[6];
@@ -354,6 +362,70 @@ __global__ __launch_bounds__(64) void k_fixup(int W, int H, int D, const float2*
         }
         store_field(field, (size_t)v, m);
     }
+}
+
+// ---------------------------------------------------------------------------
+// density-based seed re-placement (GPU-AS/Image.cpp:350-620,
+// shaders/derivative_t.gs:20-53, shaders/density_grid.vs:13-42)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float f16_round(float x) { return (float)(_Float16)x; }
+
+__device__ __forceinline__ uint32_t abs_diff_sum(uint3 a, uint3 b)
+{
+    return (a.x > b.x ? a.x - b.x : b.x - a.x) + (a.y > b.y ? a.y - b.y : b.y - a.y) +
+           (a.z > b.z ? a.z - b.z : b.z - a.z);
+}
+
+// Per voxel: g = 0.33 sum|c - c_left| + 0.33 sum|c - c_up| + 0.33
+// sum|c - c_prev| (GL rows, clamped taps; each product and sum rounded to
+// float, stored as float16 like the RGB16F target), added as an exact
+// multiple of 2^-12 into the slot that the voxel's float16 grid-seed id
+// addresses through density_grid.vs's float arithmetic (a point landing
+// outside the row is dropped, as the rasteriser would).  The grid id is the
+// separable box map (mz, my, mx).  Each thread walks 16 voxels of a row and
+// adds one atomic per run of equal slots.
+__global__ __launch_bounds__(256) void k_density(int W, int H, int D, const uint8_t* __restrict__ rgb, int prepared,
+                                                 const int* __restrict__ mx, const int* __restrict__ my,
+                                                 const int* __restrict__ mz, int wnum, int hnum, int nslot, float wr,
+                                                 unsigned long long* __restrict__ acc)
+{
+    const int segs = (W + 15) / 16;
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (gid >= (int64_t)segs * H * D) return;
+    const int seg = (int)(gid % segs);
+    const int y = (int)((gid / segs) % H);
+    const int t = (int)(gid / ((int64_t)segs * H));
+    const int x0 = seg * 16, x1 = min(W, x0 + 16);
+    if (mz[t] < 0 || my[y] < 0) return;
+    const int row_id = (mz[t] * hnum + my[y]) * wnum;
+    int cur = -1;
+    unsigned long long run = 0ull;
+    for (int x = x0; x < x1; ++x) {
+        const uint3 c = frame_rgb(rgb, prepared, W, H, x, y, t);
+        const uint32_t s1 = abs_diff_sum(c, frame_rgb(rgb, prepared, W, H, x > 0 ? x - 1 : 0, y, t));
+        const uint32_t s2 = abs_diff_sum(c, frame_rgb(rgb, prepared, W, H, x, y > 0 ? y - 1 : 0, t));
+        const uint32_t s3 = abs_diff_sum(c, frame_rgb(rgb, prepared, W, H, x, y, t > 0 ? t - 1 : 0));
+        float g = __fmul_rn(0.33f, (float)s1);
+        g = __fadd_rn(g, __fmul_rn(0.33f, (float)s2));
+        g = __fadd_rn(g, __fmul_rn(0.33f, (float)s3));
+        const float pg = f16_round(g);
+        const float kr = f16_round((float)(row_id + mx[x]));
+        const float ty = floorf(__fmul_rn(kr, wr));
+        const float tx = floorf(__fsub_rn(kr, __fdiv_rn(ty, wr)));
+        int slot = -1;
+        if (mx[x] >= 0 && tx >= 0.0f && tx < (float)W) {
+            slot = (int)ty * W + (int)tx;
+            if (slot >= nslot) slot = -1;
+        }
+        if (slot != cur) {
+            if (cur >= 0 && run) atomicAdd(acc + cur, run);
+            cur = slot;
+            run = 0ull;
+        }
+        run += (unsigned long long)__fmul_rn(pg, 4096.0f);
+    }
+    if (cur >= 0 && run) atomicAdd(acc + cur, run);
 }
 
 // ---------------------------------------------------------------------------
@@ -1368,6 +1440,18 @@ hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const 
                         float4* field, const int* fix_list, const int* fix_count)
 {
     hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), 0, st, d.W, d.H, d.D, fwd, bwd, field, fix_list, fix_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared, const int* mx,
+                          const int* my, const int* mz, int wnum, int hnum, int nslot, unsigned long long* acc)
+{
+    const int64_t threads = (int64_t)((d.W + 15) / 16) * d.H * d.D;
+    const int64_t blocks = (threads + 255) / 256;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const float wr = 1.0f / (float)d.W;      // width_ratio (GPU-AS/Image.cpp)
+    hipLaunchKernelGGL(k_density, dim3((unsigned)blocks), dim3(256), 0, st, d.W, d.H, d.D, rgb, prepared, mx, my, mz,
+                       wnum, hnum, nslot, wr, acc);
     return hipGetLastError();
 }
 

@@ -82,6 +82,11 @@ bool Image::check_dead_seeds(int* labels)
     return true;
 }
 
+bool Image::initialization(bool density)
+{
+    return ok(asv_initialization(ctx, density ? ASV_INIT_DENSITY : ASV_INIT_GRID), "initialization");
+}
+
 bool Image::enforce_connectivity_3d(int* labels, int& label_num)
 {
     int32_t n = 0;

@@ -21,6 +21,10 @@ public:
     // on the device); data_prepared = true when they were already applied.
     bool init_gpu(std::string& dbname, unsigned char* data, int width, int height, int depth, int seeds_num,
                   int Lloyd_max_iter, float compactness, bool data_prepared = false);
+    // GPU-AS/Image.cpp:350-620: density-based re-placement of the grid seeds
+    // (applied when seed_matrix / RGB2LAB prepare the clip); false keeps the
+    // plain grid.
+    bool initialization(bool density = true);
     // GPU-AS/Image.cpp:643-686: the flow fields, here ground truth (D x H x W
     // x 2 float, image rows) in place of Farneback.
     bool forward_backward_flow(const float* flow_forward, const float* flow_backward);

@@ -45,7 +45,7 @@ void mkdirs(const std::string& path)
 
 Render::Render()
     : Lloyd_ends(false), Lloyd_curr_iter(0), Lloyd_max_iter(20), width(0), height(0), depth(0),
-      output_label(true), output_render(true), connect(false), ms_total(0), ms_loop(0)
+      output_label(true), output_render(true), connect(false), density_init(false), ms_total(0), ms_loop(0)
 {
 }
 
@@ -105,6 +105,7 @@ int Render::run_arrays(const uint8_t* rgb, const float* fwd, const float* bwd, i
     // GPU-AS/Render.cpp:256-274
     if (!image.init_gpu(db, const_cast<unsigned char*>(rgb), W, H, D, seeds_num, Lloyd_max_iter, compactness))
         return -1;
+    if (density_init && !image.initialization(true)) return -1;          // GPU-AS/Render.cpp:258
     if (!image.forward_backward_flow(fwd, bwd)) return -1;
     if (!image.seed_matrix() || !image.RGB2LAB()) return -1;
     // GPU-AS/Render.cpp:277-293

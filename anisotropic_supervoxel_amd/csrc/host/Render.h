@@ -35,5 +35,6 @@ public:
     bool output_label;       // write the label text (the reference hard-codes false)
     bool output_render;      // write the pseudo-colour frames
     bool connect;            // 3D connectivity in place of check_dead_seeds (the reference's step 6.2)
+    bool density_init;       // density-based seed re-placement (the reference's step 2, Render.cpp:258)
     double ms_total, ms_loop;
 };

@@ -5,7 +5,7 @@
 //   asv --dataset DIR --video NAME
 //   common: --K 500 --iters 20 --compactness 8 --save DIR --db NAME
 //           --no-txt --no-render --eval --connect (3D connectivity instead of
-//           check_dead_seeds)
+//           check_dead_seeds) --density (density-based seed placement)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +19,7 @@ static void usage()
 {
     std::printf("usage: asv (--synth SEED,W,H,D,E,VMAX,SIGMA | --dataset DIR --video NAME) [--K n] [--iters n]\n"
                 "           [--compactness c] [--save DIR] [--db NAME] [--write-clip DIR] [--no-txt] [--no-render]\n"
-                "           [--eval] [--connect]\n");
+                "           [--eval] [--connect] [--density]\n");
 }
 
 int main(int argc, char** argv)
@@ -27,7 +27,7 @@ int main(int argc, char** argv)
     std::string synth, dataset, video, save = "results/", db = "synthetic", write_clip;
     int K = 500, iters = 20;
     float compactness = 8.0f;
-    bool txt = true, render = true, eval = false, connect = false;
+    bool txt = true, render = true, eval = false, connect = false, density = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() -> std::string {
@@ -47,6 +47,7 @@ int main(int argc, char** argv)
         else if (a == "--no-render") render = false;
         else if (a == "--eval") eval = true;
         else if (a == "--connect") connect = true;
+        else if (a == "--density") density = true;
         else { usage(); return 2; }
     }
     if (!save.empty() && save.back() != '/') save += "/";
@@ -55,6 +56,7 @@ int main(int argc, char** argv)
     r.output_label = txt;
     r.output_render = render;
     r.connect = connect;
+    r.density_init = density;
     if (!synth.empty()) {
         unsigned long long seed;
         int W, H, D, E;

@@ -33,6 +33,7 @@ enum {
 
 enum { ASV_TIE_LOWEST_ID = 0, ASV_TIE_SCAN_ORDER = 1 };
 enum { ASV_SCHED_REFERENCE = 0, ASV_SCHED_HALVING_PLUS_ONE = 1 };
+enum { ASV_INIT_GRID = 0, ASV_INIT_DENSITY = 1 };
 
 typedef struct asv_params {
     int32_t width, height, depth;   /* W, H, D (frames) */
@@ -45,6 +46,10 @@ typedef struct asv_params {
     int32_t schedule;               /* ASV_SCHED_* */
     int32_t frames_prepared;        /* 0: raw frames, image row order (row 0 = top);
                                        1: already blurred and in GL row order */
+    int32_t seed_init;              /* ASV_INIT_GRID: the regular grid (GPU-AS/Image.cpp:20-92);
+                                       ASV_INIT_DENSITY: the grid re-placed by colour-gradient
+                                       density (Image::initialization, :350-620), which the
+                                       reference's Render::run always applies (Render.cpp:258) */
 } asv_params;
 
 typedef struct asv_ctx asv_ctx;
@@ -172,6 +177,14 @@ int asv_postprocess_device(asv_ctx* ctx, const int64_t* sums, int32_t* dead_out,
  * GL raster order, for a label volume produced elsewhere.  Synchronises. */
 int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
                            int32_t* dead_out, int32_t* final_label_num, void* stream);
+
+/* Image::initialization (GPU-AS/Image.cpp:350-620): selects the initial
+ * seed placement, ASV_INIT_GRID or ASV_INIT_DENSITY (the grid re-placed by
+ * colour-gradient density), for the next asv_prepare; the same as creating
+ * the context with p->seed_init.  Contexts created from explicit seeds
+ * (asv_create_seeded, asv_reseed) have no reference grid and refuse
+ * ASV_INIT_DENSITY. */
+int asv_initialization(asv_ctx* ctx, int32_t seed_init);
 
 /* 3D connectivity enforcement (Image::enforce_connectivity_3d,
  * GPU-AS/Image.cpp:1808-1837, over EnforceLabelConnectivity3D :1938-2062)

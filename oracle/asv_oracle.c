@@ -864,3 +864,140 @@ ORC_API int orc_connectivity(const int32_t* labels, int W, int H, int D, int sup
     free(queue);
     return next;
 }
+
+/* Nearest float16 value of x (0 <= x < 65520), ties to even: the rounding
+ * of a float written to the reference's RGB16F / R16F textures. */
+static float half_round(float x)
+{
+    if (x == 0.0f) return 0.0f;
+    int e;
+    frexpf(x, &e);                        /* x = f * 2^e, 0.5 <= f < 1 */
+    int q = e - 11;                       /* 11 significant bits */
+    if (q < -24) q = -24;                 /* float16 subnormal spacing */
+    float scaled = ldexpf(x, -q);
+    return ldexpf(nearbyintf(scaled), q);
+}
+ORC_API float orc_half_round(float x) { return half_round(x); }
+
+/* dNum, hNum, wNum of the grid (GPU-AS/Image.cpp:37-45). */
+static int grid_dims(int W, int H, int D, int K, int* dn, int* hn, int* wn)
+{
+    int dNum = (int)pow((float)(K * D * D / W / H), 1.0 / 3.0);
+    if (dNum == 0) dNum = 1;
+    int fs = K / dNum;
+    int wNum = (int)sqrt((float)(fs * W / H));
+    if (wNum <= 0) return -1;
+    int hNum = fs / wNum;
+    if (hNum <= 0) return -1;
+    *dn = dNum; *hn = hNum; *wn = wNum;
+    return dNum * hNum * wNum;
+}
+
+/* Box of grid index i along one axis of n seeds at (integer) positions p,
+ * GPU-AS/Image.cpp:360-433: from floor(0.5 (p[i-1]+.5 + p[i]+.5)) (0 for the
+ * first) to floor(0.5 (p[i+1]+.5 + p[i]+.5)) (N-1 for the last). */
+static void axis_box(const int* p, int n, int N, int i, int* lo, int* hi)
+{
+    *lo = i > 0 ? (int)floor(0.5 * (((double)p[i - 1] + 0.5) + ((double)p[i] + 0.5))) : 0;
+    *hi = i < n - 1 ? (int)floor(0.5 * (((double)p[i + 1] + 0.5) + ((double)p[i] + 0.5))) : N - 1;
+}
+
+/* Density-based re-placement of the grid seeds, GPU-AS/Image.cpp:350-620
+ * (Image::initialization) with derivative_t.gs and density_grid.vs:
+ *  1. every grid seed owns the box between the midpoints to its grid
+ *     neighbours; a voxel in two boxes belongs to the later seed (the grid
+ *     volume is painted in seed order), stored as float16 (R16F);
+ *  2. per voxel g = 0.33 sum|c - c_left| + 0.33 sum|c - c_up| + 0.33
+ *     sum|c - c_prev| over R, G, B of the blurred frames (GL rows: up is
+ *     y - 1, prev is t - 1; clamped taps at the edges), each product and sum
+ *     rounded to float, the result stored as float16 (RGB16F);
+ *  3. density of slot k = 3 (the clear colour) + the sum of g over voxels
+ *     whose float16 seed id addresses texel k of slice 0 by density_grid.vs's
+ *     float arithmetic; the reference adds in blend order, this sums exactly
+ *     and rounds once to float;
+ *  4. densities 0.001 * d (double product, stored as float), float prefix
+ *     sums, A seeds at (i + 1) * total / (A + 1) along them; each takes the
+ *     y and z of the grid seed whose interval holds it and an x interpolated
+ *     across that seed's x box.
+ * img: D x H x W x 3 frames in GL row order (blurred).  Writes A x 3 voxel
+ * positions (x, y, z) and, if dens is not NULL, the A slot densities
+ * (step 3).  Returns A, or a negative value on bad input. */
+ORC_API int orc_density_seeds(const uint8_t* img, int W, int H, int D, int K, int* xyz, float* dens)
+{
+    int dN, hN, wN;
+    const int A = grid_dims(W, H, D, K, &dN, &hN, &wN);
+    if (A <= 0) return -1;
+    int* g = (int*)malloc(sizeof(int) * 3 * (size_t)A);
+    int s = 0;
+    if (orc_grid_seeds(W, H, D, K, g, &s) != A) { free(g); return -2; }
+    int* px = (int*)malloc(sizeof(int) * (size_t)wN);
+    int* py = (int*)malloc(sizeof(int) * (size_t)hN);
+    int* pz = (int*)malloc(sizeof(int) * (size_t)dN);
+    for (int i = 0; i < wN; ++i) px[i] = g[3 * i];
+    for (int j = 0; j < hN; ++j) py[j] = g[3 * (j * wN) + 1];
+    for (int t = 0; t < dN; ++t) pz[t] = g[3 * (t * wN * hN) + 2];
+    int* mx = (int*)malloc(sizeof(int) * (size_t)W);
+    int* my = (int*)malloc(sizeof(int) * (size_t)H);
+    int* mz = (int*)malloc(sizeof(int) * (size_t)D);
+    for (int x = 0; x < W; ++x) mx[x] = -1;
+    for (int y = 0; y < H; ++y) my[y] = -1;
+    for (int z = 0; z < D; ++z) mz[z] = -1;
+    for (int i = 0; i < wN; ++i) { int lo, hi; axis_box(px, wN, W, i, &lo, &hi); for (int x = lo; x <= hi; ++x) mx[x] = i; }
+    for (int j = 0; j < hN; ++j) { int lo, hi; axis_box(py, hN, H, j, &lo, &hi); for (int y = lo; y <= hi; ++y) my[y] = j; }
+    for (int t = 0; t < dN; ++t) { int lo, hi; axis_box(pz, dN, D, t, &lo, &hi); for (int z = lo; z <= hi; ++z) mz[z] = t; }
+
+    const int nslot = (int)half_round((float)(A - 1)) + 1;
+    int64_t* acc = (int64_t*)calloc((size_t)nslot, sizeof(int64_t));
+    const float wr = 1.0f / (float)W;
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const uint8_t* c = img + (((size_t)z * H + y) * W + x) * 3;
+                const uint8_t* nb[3] = {img + (((size_t)z * H + y) * W + (x > 0 ? x - 1 : 0)) * 3,
+                                        img + (((size_t)z * H + (y > 0 ? y - 1 : 0)) * W + x) * 3,
+                                        img + (((size_t)(z > 0 ? z - 1 : 0) * H + y) * W + x) * 3};
+                float grad = 0.0f;
+                for (int q = 0; q < 3; ++q) {
+                    int sum = abs((int)nb[q][0] - (int)c[0]) + abs((int)nb[q][1] - (int)c[1]) +
+                              abs((int)nb[q][2] - (int)c[2]);
+                    grad = grad + 0.33f * (float)sum;
+                }
+                const float pg = half_round(grad);
+                if (mx[x] < 0 || my[y] < 0 || mz[z] < 0) continue;
+                const int k = (mz[z] * hN + my[y]) * wN + mx[x];
+                const float kr = half_round((float)k);
+                const float ty = floorf(kr * wr);
+                const float tx = floorf(kr - ty / wr);
+                if (tx < 0.0f || tx >= (float)W) continue;
+                const int slot = (int)ty * W + (int)tx;
+                if (slot < 0 || slot >= nslot) continue;
+                acc[slot] += (int64_t)(pg * 4096.0f);
+            }
+    float* len = (float*)malloc(sizeof(float) * (size_t)A);
+    for (int i = 0; i < A; ++i) {
+        const float res = (float)(3.0 + (double)acc[i] / 4096.0);
+        if (dens) dens[i] = res;
+        const float d = (float)(0.001 * (double)res);
+        len[i] = i == 0 ? d : d + len[i - 1];
+    }
+    const float interval = len[A - 1] / (float)(A + 1);
+    int j1 = 0, placed = 0;
+    for (int i = 0; i < A;) {
+        const float sin_ = (float)(i + 1) * interval;
+        if (sin_ < len[j1]) {
+            int lo, hi;
+            axis_box(px, wN, W, j1 % wN, &lo, &hi);
+            const float frac = j1 == 0 ? sin_ / len[j1] : (sin_ - len[j1 - 1]) / (len[j1] - len[j1 - 1]);
+            const float ax = floorf((float)lo + (float)(hi - lo) * frac);
+            xyz[3 * i] = (int)ax;
+            xyz[3 * i + 1] = g[3 * j1 + 1];
+            xyz[3 * i + 2] = g[3 * j1 + 2];
+            ++i;
+            ++placed;
+        } else if (++j1 == A) {
+            break;
+        }
+    }
+    free(g); free(px); free(py); free(pz); free(mx); free(my); free(mz); free(acc); free(len);
+    return placed == A ? A : -3;
+}

@@ -60,6 +60,8 @@ def load():
             "orc_run": (i, [vp, vp, vp, vp, vp, vp, vp]),
             "orc_seed_record_floats": (i, []),
             "orc_connectivity": (i, [vp, i, i, i, i, vp]),
+            "orc_half_round": (f, [f]),
+            "orc_density_seeds": (i, [vp, i, i, i, i, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -220,3 +222,24 @@ def connectivity(labels, supsz):
     if n < 0:
         raise MemoryError("orc_connectivity")
     return out, n
+
+
+def half_round(x):
+    return load().orc_half_round(float(x))
+
+
+def density_seeds(rgb, K, prepared=0):
+    """Image::initialization (GPU-AS/Image.cpp:350-620) restated: the grid
+    seeds re-placed by colour-gradient density.  rgb: D x H x W x 3 raw
+    frames (blurred and flipped here unless prepared).  Returns (A x 3 voxel
+    positions, A slot densities)."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    img = rgb if prepared else blur_flip(rgb)
+    D, H, W, _ = img.shape
+    A = load().orc_grid_seeds(W, H, D, K, None, None)
+    xyz = np.zeros((max(A, 1), 3), np.int32)
+    dens = np.zeros(max(A, 1), np.float32)
+    r = load().orc_density_seeds(_p(img), W, H, D, K, _p(xyz), _p(dens))
+    if r < 0:
+        raise ValueError(f"orc_density_seeds failed: {r}")
+    return xyz[:r], dens[:r]
