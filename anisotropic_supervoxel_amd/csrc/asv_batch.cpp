@@ -127,12 +127,16 @@ int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
     for (int i : g.clips) {
         const asv_ctx* c = b->clips[(size_t)i];
         const float ratio = c->mult[(size_t)g.p] * c->ratio;
-        if (g.p == 0 && g.K <= kSeedPassMaxStep)      // the seed stage has just scattered the seeds
-            BHIP(launch_jfa_seeds(st, c->d, c->labels[0], c->labels[1], c->lab, c->rec, c->pos, g.K, ratio,
-                                  c->p.tie_rule));
-        else
-            BHIP(launch_jfa(st, c->d, c->labels[g.p & 1], c->labels[1 - (g.p & 1)], c->lab, c->rec, g.K, ratio,
-                            c->p.tie_rule));
+        const int32_t* in = c->labels[g.p & 1];
+        int32_t* out = c->labels[1 - (g.p & 1)];
+        switch (c->pass_kind((size_t)g.p)) {       // every iteration starts with the seed stage
+        case 1: BHIP(launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, g.K, ratio, c->p.tie_rule)); break;
+        case 2:
+            BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
+                                   c->keys));
+            break;
+        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule)); break;
+        }
     }
     return ASV_OK;
 }
@@ -254,8 +258,9 @@ int asv_batch_create(const asv_params* params, int32_t n, asv_batch** out)
             const asv_ctx* c = b->clips[(size_t)i];
             if (p >= (int)c->steps.size()) continue;
             const int K = c->steps[(size_t)p];
-            // pass 0 runs per clip over the scattered seeds only
-            const bool march = p > 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID && c->d.A <= kMarchMaxIds;
+            // passes over the seeds' neighbourhoods alone run per clip
+            const bool march = c->pass_kind((size_t)p) == 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID &&
+                               c->d.A <= kMarchMaxIds;
             PassGroup* g = nullptr;
             if (march)
                 for (PassGroup& q : row)

@@ -150,14 +150,24 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace, c->map_table, c->post_scratch, c->ck_word};
+                    c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
 
-int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace)
+// Whether a context keeps the keys of the scatter-form second pass: the
+// lowest-id rule, a sparse second pass, coordinates that fit 16 bits.
+bool wants_keys(const asv_params* p, const std::vector<int>& steps, int64_t A)
+{
+    const int64_t V = (int64_t)p->width * p->height * p->depth;
+    return p->tie_rule == ASV_TIE_LOWEST_ID && seed_pass_kind(steps, A, V, 1) == 2 && p->width < 32768 &&
+           p->height < 32768 && p->depth < 32768;
+}
+
+int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace, bool keys)
 {
     int64_t b = 0;
+    if (keys) b += V * 8;        // scatter keys of the sparse second pass
     b += V * 3;                  // rgb
     b += V * 16;                 // fwd + bwd
     b += V * 4;                  // lab
@@ -214,7 +224,10 @@ int64_t asv_footprint(const asv_params* p)
     int s = 0;
     int A = grid_seeds(p->width, p->height, p->depth, p->seeds_num, nullptr, &s);
     if (A <= 0) return -1;
-    return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false);
+    std::vector<int> st;
+    std::vector<float> mu;
+    schedule_list(p->schedule, s < 1 ? 1 : s, p->width, st, mu);
+    return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false, wants_keys(p, st, A));
 }
 
 int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step, int32_t* steps, float* mult,
@@ -277,6 +290,10 @@ int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out
     if (e == hipSuccess) e = dalloc(&c->rec, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(Acap + 1) * kSumWords, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(Acap + 1) * kSumWords, &tb);
+    if (e == hipSuccess && wants_keys(p, c->steps, A)) {
+        e = dalloc(&c->keys, (size_t)V, &tb);
+        if (e == hipSuccess) e = launch_fill_keys(nullptr, c->keys, V);
+    }
     if (e != hipSuccess) {
         free_ctx(c);
         delete c;
@@ -589,17 +606,24 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
 
 namespace {
 
-// Pass i of the schedule from labels[cur] into labels[1 - cur].  The first
-// pass after the seed stage sees only the scattered seeds and, for steps up
-// to kSeedPassMaxStep, evaluates just the voxels around them.
+// Pass i of the schedule from labels[cur] into labels[1 - cur].  Right after
+// the seed stage the first pass sees only the scattered seeds and the second
+// only their neighbourhoods, so those evaluate just the voxels that can
+// carry a label (asv_ctx::pass_kind).
+int pass_kind_now(const asv_ctx* c, size_t i) { return c->fresh ? c->pass_kind(i) : 0; }
+
 hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
 {
     const float ratio = c->mult[i] * c->ratio;
-    if (i == 0 && c->fresh && c->steps[0] <= kSeedPassMaxStep)
-        return launch_jfa_seeds(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->pos,
-                                c->steps[0], ratio, c->p.tie_rule);
-    return launch_jfa(st, c->d, c->labels[c->cur], c->labels[1 - c->cur], c->lab, c->rec, c->steps[i], ratio,
-                      c->p.tie_rule);
+    const int32_t* in = c->labels[c->cur];
+    int32_t* out = c->labels[1 - c->cur];
+    switch (pass_kind_now(c, i)) {
+    case 1: return launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], ratio, c->p.tie_rule);
+    case 2:
+        return launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], c->steps[1], ratio,
+                                 c->p.tie_rule, c->keys);
+    default: return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule);
+    }
 }
 
 }  // namespace
@@ -677,7 +701,7 @@ int asv_run_timed(asv_ctx* c, void* stream, float* ms, int32_t* passes)
         spans.push_back({a, mark(), 4});
         for (size_t i = 0; i < c->steps.size(); ++i) {
             a = mark();
-            const bool first = i == 0 && c->fresh && c->steps[0] <= kSeedPassMaxStep;
+            const bool first = pass_kind_now(c, i) != 0;
             HIP_TRY(launch_pass(c, i, st));
             spans.push_back({a, mark(), first ? 5 : 2});
             c->cur = 1 - c->cur;

@@ -102,10 +102,16 @@ __global__ __launch_bounds__(256) void k_cc_union(int W, int H, int D, const int
     }
 }
 
+// Every voxel points at its root.  The walk only reads: a path-halving
+// store of one thread could otherwise overwrite, with a mere ancestor, the
+// root another thread has just written.
 __global__ __launch_bounds__(256) void k_cc_flatten(int64_t V, int* parent)
 {
-    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (int64_t)gridDim.x * 256)
-        parent[v] = cc_root(parent, (int)v);
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < V; v += (int64_t)gridDim.x * 256) {
+        int cur = (int)v, next;
+        while ((next = ld_parent(parent, cur)) != cur) cur = next;
+        __hip_atomic_store(parent + v, cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Roots per chunk of kCcChunk voxels.

@@ -45,8 +45,15 @@ struct asv_ctx {
     long long* post_scratch = nullptr;
     int64_t post_cap = 0;
     unsigned long long* ck_word = nullptr;
+    unsigned long long* keys = nullptr;     // V keys for the scatter form of the sparse second pass
 
     int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
     int depth_cap = 0;      // frames the volume buffers hold
     int64_t V() const { return (int64_t)d.W * d.H * d.D; }
+
+    // How pass i of an iteration that began with the seed stage runs: 1 the
+    // first pass over the scattered seeds alone, 2 the second pass over the
+    // first's output when it is sparse (about 729 voxels per seed against
+    // a third of the volume), 0 the full marching/tiled pass.
+    int pass_kind(size_t i) const { return asv::seed_pass_kind(steps, d.A, V(), i); }
 };

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace asv {
 
 // One seed as the flood reads it: 48 bytes, three 16-byte loads.
@@ -66,6 +68,31 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 // and evaluates only the voxels that have a seed among their taps, at most
 // max(3, k + 1)^3 per seed; used for steps k <= kSeedPassMaxStep.
 constexpr int kSeedPassMaxStep = 16;
+// The second pass (step k2) over the first pass's output (step k1 from the
+// scattered seeds): fills `out` with -1 and evaluates only the voxels that
+// can have a labelled tap, at most (3 (k2 + 1))^3 per seed (9^3 inside).
+// With `keys` (V 64-bit words, all at the initial key, as
+// launch_fill_keys leaves them and this pass returns them) the lowest-id
+// rule runs by scatter: one atomic minimum per (labelled tap, voxel) pair.
+hipError_t launch_jfa_seeds2(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                             const SeedRec* rec, const int4* pos, int k1, int k2, float ratio, int tie_rule,
+                             unsigned long long* keys);
+hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n);
+
+// How pass i of an iteration that began with the seed stage runs, for a
+// schedule `steps`, A seeds and V voxels: 1 the first pass over the
+// scattered seeds alone, 2 the second pass over the first's output when it
+// is sparse (about 729 voxels per seed against a third of the volume), 0
+// the full marching/tiled pass.
+inline int seed_pass_kind(const std::vector<int>& steps, int64_t A, int64_t V, size_t i)
+{
+    if (steps.empty()) return 0;
+    if (i == 0 && steps[0] <= kSeedPassMaxStep) return 1;
+    if (i == 1 && steps.size() > 1 && steps[0] <= kSeedPassMaxStep && steps[1] <= kSeedPassMaxStep &&
+        A * 729 * 3 <= V)
+        return 2;
+    return 0;
+}
 hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
                             const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule);
 // The marching pass over several clips at once (step k <= 16, lowest-id

@@ -729,6 +729,133 @@ __global__ __launch_bounds__(256) void k_jfa_seeds(int W, int H, int D, int A, i
     out[((size_t)z * H + y) * W + x] = jfa_voxel<TIE>(W, H, D, in, lab, rec, k, ratio, x, y, z);
 }
 
+// Second pass of an iteration, over the first pass's output (step k1 from
+// the seeds alone): a voxel there carries a label only if every coordinate
+// lies in seed_axis's set of its seed (step k1), so a voxel of this pass
+// (step k2) has a labelled tap only if every coordinate lies in the union,
+// over that set, of seed_axis's sets for step k2 -- at most 3 (k2 + 1)
+// values per axis around each seed, 9 inside the volume.  One block per seed
+// lists the three axis sets in LDS and evaluates their product exactly as
+// the full pass does; every other voxel keeps the -1 the output was filled
+// with.  Voxels shared by neighbouring seeds are written by each with the
+// same value.
+constexpr int kSeed2Span = 2 * 2 * kSeedPassMaxStep + 1;   // coordinates within k1 + k2 of the seed
+
+__device__ __forceinline__ bool axis_hit(int c, int a, int n, int k)
+{
+    return clampi(c - k, 0, n - 1) == a || c == a || clampi(c + k, 0, n - 1) == a;
+}
+
+template <int TIE>
+__global__ __launch_bounds__(256) void k_jfa_seeds2(int W, int H, int D, int k1, int k2,
+                                                    const int4* __restrict__ pos, const int32_t* __restrict__ in,
+                                                    int32_t* __restrict__ out, const uint32_t* __restrict__ lab,
+                                                    const SeedRec* __restrict__ rec, float ratio)
+{
+    __shared__ int s_v[3][kSeed2Span];
+    __shared__ int s_n[3];
+    const int4 p = pos[blockIdx.x];
+    const int r = k1 + k2;
+    if (threadIdx.x < 3) {
+        const int ax = threadIdx.x;
+        const int pa = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+        const int n = ax == 0 ? W : (ax == 1 ? H : D);
+        int cnt = 0;
+        for (int c = max(0, pa - r); c <= min(n - 1, pa + r); ++c) {
+            // c reaches, at step k2, some a that reaches the seed at step k1
+            bool in_set = false;
+            for (int a = max(0, pa - k1); a <= min(n - 1, pa + k1) && !in_set; ++a)
+                in_set = axis_hit(a, pa, n, k1) && axis_hit(c, a, n, k2);
+            if (in_set) s_v[ax][cnt++] = c;
+        }
+        s_n[ax] = cnt;
+    }
+    __syncthreads();
+    const int nx = s_n[0], ny = s_n[1], nz = s_n[2];
+    const int total = nx * ny * nz;
+    for (int j = threadIdx.x; j < total; j += 256) {
+        const int x = s_v[0][j % nx], y = s_v[1][(j / nx) % ny], z = s_v[2][j / (nx * ny)];
+        out[((size_t)z * H + y) * W + x] = jfa_voxel<TIE>(W, H, D, in, lab, rec, k2, ratio, x, y, z);
+    }
+}
+
+// The same second pass for the lowest-id rule, by scatter instead of gather:
+// every label the first pass can have written sits at a voxel t whose
+// coordinates lie in seed_axis's step-k1 sets of a seed, and a voxel v has
+// t among its step-k2 taps exactly when each coordinate of v lies in
+// seed_axis's step-k2 set of t's.  One block per seed lists, per axis, the
+// (t, v) coordinate pairs, and their product gives every (labelled tap,
+// voxel) pair of the seed's neighbourhood.  mode 0: each pair offers its
+// candidate to v's 64-bit key (distance bits, id) by atomic minimum --
+// the minimum over v's labelled taps is the full pass's lowest-id winner,
+// and repeats (a tap reached twice, or from a neighbouring seed's block)
+// cannot change a minimum; mode 1: v takes the label of its key (-1 while
+// the key is still the initial one, d >= 1e8); mode 2: the keys touched
+// go back to the initial value for the next use (a separate launch: a
+// repeat of v must still read its key in mode 1; folding the reset into
+// mode 1 as an atomic exchange measured 2x slower than the two plain
+// launches).  A tap cell is read once per pair, instead of 27 scattered
+// taps per voxel.
+constexpr int kSeed2Pairs = (kSeedPassMaxStep + 1) * (kSeedPassMaxStep + 1);
+
+__global__ __launch_bounds__(256) void k_jfa_seeds2_scatter(int W, int H, int D, int k1, int k2, int mode,
+                                                            const int4* __restrict__ pos,
+                                                            const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                            const uint32_t* __restrict__ lab,
+                                                            const SeedRec* __restrict__ rec, float ratio,
+                                                            unsigned long long* __restrict__ keys)
+{
+    __shared__ short s_t[3][kSeed2Pairs];
+    __shared__ short s_v[3][kSeed2Pairs];
+    __shared__ int s_n[3];
+    const int4 p = pos[blockIdx.x];
+    if (threadIdx.x < 3) {
+        const int ax = threadIdx.x;
+        const int pa = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+        const int n = ax == 0 ? W : (ax == 1 ? H : D);
+        int cnt = 0;
+        for (int a = max(0, pa - k1); a <= min(n - 1, pa + k1); ++a) {
+            if (!axis_hit(a, pa, n, k1)) continue;
+            for (int c = max(0, a - k2); c <= min(n - 1, a + k2); ++c)
+                if (axis_hit(c, a, n, k2)) {
+                    s_t[ax][cnt] = (short)a;
+                    s_v[ax][cnt] = (short)c;
+                    ++cnt;
+                }
+        }
+        s_n[ax] = cnt;
+    }
+    __syncthreads();
+    const int nx = s_n[0], ny = s_n[1], nz = s_n[2];
+    const int total = nx * ny * nz;
+    for (int j = threadIdx.x; j < total; j += 256) {
+        const int ix = j % nx, iy = (j / nx) % ny, iz = j / (nx * ny);
+        const size_t v = ((size_t)s_v[2][iz] * H + s_v[1][iy]) * W + s_v[0][ix];
+        if (mode == 0) {
+            const int32_t id = in[((size_t)s_t[2][iz] * H + s_t[1][iy]) * W + s_t[0][ix]];
+            if (id < 0) continue;
+            const uint32_t pk = lab[v];
+            const SeedRec r = rec[id];
+            const float d = cand_dist(r.a, r.b, r.c, (float)s_v[0][ix], (float)s_v[1][iy], (float)s_v[2][iz],
+                                      (float)(pk & 255u), (float)((pk >> 8) & 255u), (float)((pk >> 16) & 255u),
+                                      ratio);
+            const unsigned long long kk = dkey(d, id);
+            if (kk < ASV_KEY_INIT) atomicMin(keys + v, kk);
+        } else if (mode == 1) {
+            const unsigned long long kk = keys[v];
+            out[v] = kk >= ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(kk & 0xffffffffull);
+        } else {
+            keys[v] = ASV_KEY_INIT;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict__ p, int64_t n,
+                                                  unsigned long long value)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = value;
+}
+
 // Tiled pass for the lowest-id rule and step K (template), 16 x 16 voxels of
 // one slice per block, four 8 x 8 waves.
 //  1. The labels of the three slabs (z, z-K, z+K) over the tile grown by K
@@ -1690,6 +1817,34 @@ hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, in
     else
         hipLaunchKernelGGL(k_jfa_seeds<1>, dim3((unsigned)blocks), dim3(256), 0, st, d.W, d.H, d.D, d.A, k, m, pos, in,
                            out, lab, rec, ratio);
+    return hipGetLastError();
+}
+
+hipError_t launch_jfa_seeds2(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
+                             const SeedRec* rec, const int4* pos, int k1, int k2, float ratio, int tie_rule,
+                             unsigned long long* keys)
+{
+    if (k1 < 1 || k2 < 1 || k1 > kSeedPassMaxStep || k2 > kSeedPassMaxStep) return hipErrorInvalidValue;
+    hipError_t e = launch_fill(st, out, (int64_t)d.W * d.H * d.D, -1);
+    if (e != hipSuccess || d.A < 1) return e;
+    if (tie_rule == 0 && keys) {
+        for (int mode = 0; mode < 3; ++mode)
+            hipLaunchKernelGGL(k_jfa_seeds2_scatter, dim3((unsigned)d.A), dim3(256), 0, st, d.W, d.H, d.D, k1, k2, mode,
+                               pos, in, out, lab, rec, ratio, keys);
+    } else if (tie_rule == 0)
+        hipLaunchKernelGGL(k_jfa_seeds2<0>, dim3((unsigned)d.A), dim3(256), 0, st, d.W, d.H, d.D, k1, k2, pos, in, out,
+                           lab, rec, ratio);
+    else
+        hipLaunchKernelGGL(k_jfa_seeds2<1>, dim3((unsigned)d.A), dim3(256), 0, st, d.W, d.H, d.D, k1, k2, pos, in, out,
+                           lab, rec, ratio);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(b < 65536 ? (b > 0 ? b : 1) : 65536)), dim3(256), 0, st, keys, n,
+                       ASV_KEY_INIT);
     return hipGetLastError();
 }
 

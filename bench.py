@@ -9,8 +9,8 @@ reference JFA schedule 1,9,5,3,2,1,2,1 (P = 8), FP32 metric, int32 labels.
 
 Prints one JSON line.  `value` is Mvoxel-iters/s = V * iters * steps / time.
 `roofline` describes the dominant kernel (one full JFA pass, the marching
-kernel: every pass of an iteration but the first, which runs over the
-scattered seeds only and is timed separately): 12 algorithmic bytes per
+kernel; an iteration's first pass, and its second when sparse, evaluate only
+the seeds' neighbourhoods and are timed separately): 12 algorithmic bytes per
 voxel (int32 label in, packed LAB in, int32 label out) over the pass's
 average launch time from HIP events on the launch stream.
 `cpu_baseline` times the CPU oracle on a bounded sample of the same video.
@@ -262,7 +262,7 @@ def main():
             "total_timed_run": round(float(ms[0]), 3),
             "prepare": round(float(ms[1]), 3),
             "jfa_full_passes": round(float(ms[2]), 3),
-            "jfa_first_passes": round(float(ms[5]), 3),
+            "jfa_seed_neighbourhood_passes": round(float(ms[5]), 3),
             "reduction_all": round(float(ms[3]), 3),
             "seed_and_finalize": round(float(ms[4]), 3),
             "reduction_avg": round(red_avg_ms, 4),

@@ -221,10 +221,10 @@ int asv_download_metric(asv_ctx* ctx, float* m8, void* stream);
 
 /* Timed run: same work as asv_run, with HIP events around every kernel
  * launch on stream.  ms[0] total, ms[1] prepare, ms[2] sum of the full JFA
- * passes (every pass but each iteration's first), ms[3] sum of reductions,
- * ms[4] seed stages + finalisation, ms[5] sum of each iteration's first
- * pass (over the scattered seeds only); passes receives the count of full
- * passes. */
+ * passes, ms[3] sum of reductions, ms[4] seed stages + finalisation, ms[5]
+ * sum of the passes that evaluate only the neighbourhoods of the seeds
+ * (each iteration's first, and its second when that is sparse); passes
+ * receives the count of full passes. */
 int asv_run_timed(asv_ctx* ctx, void* stream, float* ms, int32_t* passes);
 
 /* Synthetic video of the benchmark generator: background bilinear gradient,

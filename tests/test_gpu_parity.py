@@ -91,9 +91,17 @@ def test_metric_field_random_flows(have_gpu, seed):
     (8, (12, 48, 48), 2000, 3, {}),
     # partial tiles on both axes, steps above the tiled range (generic kernel)
     (9, (6, 37, 70), 6, 3, {}),
+    # sparse seeds: the second pass evaluates only the seeds' neighbourhoods
+    # (seed sets clamped at every face), for both tie rules
+    (13, (40, 80, 96), 20, 3, {}),
+    (14, (30, 64, 64), 12, 3, {"tie_rule": 1}),
+    (15, (16, 40, 200), 30, 3, {}),
 ])
 def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
     D, H, W = shape
+    if seed >= 13:
+        A, s = asv.plan(asv.default_params(W, H, D, K))[:2]
+        assert A * 729 * 3 <= W * H * D and 1 <= s // 2 <= 16, "case must take the sparse second pass"
     rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 6, 2.0, 4.0)
     got = _clip_run(rgb, fwd, bwd, K, iters, **kw)
     _assert_run_equal(got, rgb, fwd, bwd, K, iters, **{
