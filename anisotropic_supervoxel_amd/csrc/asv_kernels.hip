@@ -102,77 +102,23 @@ __device__ __constant__ int c_expected_b[9] = {3, 2, 3, 2, 1, 2, 3, 2, 3};
 // Upper triangle of N = A^T A (21 entries, row-major i<=j) and c = A^T b,
 // accumulated row by row in the reference's neighbour order (offset_y outer,
 // offset_x inner, forward row before backward row).  Entries are exact
-// integers: while every delta stays within +-73 (each |entry| of a row
-// <= 2 * 73^2 and 18 rows of products < 2^31) they are summed in 32-bit
-// integers, otherwise in double (exact below 2^53); either way the result
-// is the same exact integer.  Returns the row count.
-constexpr float kSmallDelta = 73.0f;
-
+// integers.  Returns the row count.
 __device__ __forceinline__ int normal_equations(const float2* __restrict__ fwd, const float2* __restrict__ bwd,
                                                 int W, int H, int D, int x, int y, int t,
                                                 double N[21], double c[6])
 {
-    uint32_t Ni[21], ci[6];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) Ni[i] = 0u;
+    for (int i = 0; i < 21; ++i) N[i] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) ci[i] = 0u;
+    for (int i = 0; i < 6; ++i) c[i] = 0.0;
     int rows = 0;
-    bool big = false;
     const size_t frame = (size_t)t * H * W;
     for (int m = -1; m <= 1; ++m) {
         for (int n = -1; n <= 1; ++n) {
             int nx = x + n, ny = y + m;
             if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-            const uint32_t b = (uint32_t)c_expected_b[(m + 1) * 3 + (n + 1)];
-            size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;   // image row of GL row ny
-            for (int dir = 0; dir < 2; ++dir) {
-                if (dir == 0 && t == D - 1) continue;
-                if (dir == 1 && t == 0) continue;
-                float2 f = dir == 0 ? fwd[pix] : bwd[pix];
-                float fu = f.x, fv = -f.y;
-                float dx = (float)n + floorf(fu + 0.5f);
-                float dy = (float)m + floorf(fv + 0.5f);
-                ++rows;
-                if (!(fabsf(dx) <= kSmallDelta && fabsf(dy) <= kSmallDelta)) {
-                    big = true;
-                    continue;
-                }
-                const int ix = (int)dx, iy = (int)dy, sg = dir == 0 ? 2 : -2;
-                uint32_t r[6];
-                r[0] = (uint32_t)(ix * ix);
-                r[1] = (uint32_t)(2 * ix * iy);
-                r[2] = (uint32_t)(sg * ix);
-                r[3] = (uint32_t)(iy * iy);
-                r[4] = (uint32_t)(sg * iy);
-                r[5] = 1u;
-                int q = 0;
-#pragma unroll
-                for (int i = 0; i < 6; ++i) {
-#pragma unroll
-                    for (int j = i; j < 6; ++j) { Ni[q] += r[i] * r[j]; ++q; }
-                    ci[i] += r[i] * b;
-                }
-            }
-        }
-    }
-    if (!big) {
-#pragma unroll
-        for (int i = 0; i < 21; ++i) N[i] = (double)(int32_t)Ni[i];
-#pragma unroll
-        for (int i = 0; i < 6; ++i) c[i] = (double)(int32_t)ci[i];
-        return rows;
-    }
-#pragma unroll
-    for (int i = 0; i < 21; ++i) N[i] = 0.0;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) c[i] = 0.0;
-    for (int m = -1; m <= 1; ++m) {
-        for (int n = -1; n <= 1; ++n) {
-            int nx = x + n, ny = y + m;
-            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
             double b = (double)c_expected_b[(m + 1) * 3 + (n + 1)];
-            size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;
+            size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;   // image row of GL row ny
             for (int dir = 0; dir < 2; ++dir) {
                 if (dir == 0 && t == D - 1) continue;
                 if (dir == 1 && t == 0) continue;
@@ -195,6 +141,7 @@ __device__ __forceinline__ int normal_equations(const float2* __restrict__ fwd, 
                     for (int j = i; j < 6; ++j) { N[q] += r[i] * r[j]; ++q; }
                     c[i] += r[i] * b;
                 }
+                ++rows;
             }
         }
     }
