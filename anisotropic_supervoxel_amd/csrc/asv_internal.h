@@ -69,8 +69,10 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 // max(3, k + 1)^3 per seed; used for steps k <= kSeedPassMaxStep.
 constexpr int kSeedPassMaxStep = 16;
 // The second pass (step k2) over the first pass's output (step k1 from the
-// scattered seeds): fills `out` with -1 and evaluates only the voxels that
-// can have a labelled tap, at most (3 (k2 + 1))^3 per seed (9^3 inside).
+// scattered seeds): evaluates only the voxels that can have a labelled tap,
+// at most (3 (k2 + 1))^3 per seed (9^3 inside), each seed's own voxel among
+// them.  `out` must hold -1 everywhere but at the seed voxels: the seed
+// stage's volume, which is where the schedule's second pass writes.
 // With `keys` (V 64-bit words, all at the initial key, as
 // launch_fill_keys leaves them and this pass returns them) the lowest-id
 // rule runs by scatter: one atomic minimum per (labelled tap, voxel) pair.

@@ -1825,8 +1825,9 @@ hipError_t launch_jfa_seeds2(hipStream_t st, const Dims& d, const int32_t* in, i
                              unsigned long long* keys)
 {
     if (k1 < 1 || k2 < 1 || k1 > kSeedPassMaxStep || k2 > kSeedPassMaxStep) return hipErrorInvalidValue;
-    hipError_t e = launch_fill(st, out, (int64_t)d.W * d.H * d.D, -1);
-    if (e != hipSuccess || d.A < 1) return e;
+    if (d.A < 1) return hipSuccess;
+    // `out` is the seed stage's volume: -1 except at the seed voxels, and each
+    // seed voxel lies in its own block's product set, so no fill is needed
     if (tie_rule == 0 && keys) {
         for (int mode = 0; mode < 3; ++mode)
             hipLaunchKernelGGL(k_jfa_seeds2_scatter, dim3((unsigned)d.A), dim3(256), 0, st, d.W, d.H, d.D, k1, k2, mode,
