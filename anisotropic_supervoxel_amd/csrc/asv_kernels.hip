@@ -1086,12 +1086,16 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 //    three dz terms of the distance (dz 2m2, dz 2m4, dz^2 m5) recomputed
 //    once per slice for the whole block;
 //  - two barriers per step (claims done; new slab and records staged).
+// With S = 2 a step evaluates slices z and z + K from four slab masks and
+// stages the two new slabs z + 2K and z + 3K, so the barriers, claims and
+// record staging are paid once per two slices.
 // A rebuild that still leaves more than MC candidates evaluates that slice
 // per tap from global labels and records.
-template <int K, int MC>
+template <int K, int MC, int S>
 __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
                                                    int nitems, int slot_words)
 {
+    static_assert(S == 1 || S == 2, "one or two slices of a chain per step");
     using MaskT = typename MarchMask<MC>::T;
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
     constexpr int RA = R * R;
@@ -1102,9 +1106,9 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // register costs a block per CU, more than the latency it hides
     constexpr bool kLabAhead = K != 2;
     extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
-    __shared__ uint8_t s_idx[3 * RA];
+    __shared__ uint8_t s_idx[(S + 2) * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
-    __shared__ float4 s_u[MC];                // (dz 2m2, dz 2m4, dz^2 m5, m3) at the current slice
+    __shared__ float4 s_u[S][MC];             // (dz 2m2, dz 2m4, dz^2 m5, m3) at the step's slices
     __shared__ float4 s_c[MC];                // (L, a, b, id bits)
     __shared__ float4 s_z[MC];                // (z, 2 m2, 2 m4, m5)
     __shared__ int s_list[kListCap];
@@ -1216,9 +1220,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             }
         };
         // records of list entries [from, to) from global memory, and the dz
-        // terms of every entry [0, to) for slice z
+        // terms of every entry [0, to) for the step's slices z, z + K, ...
         auto stage = [&](int from, int to, int z) {
-            const float fz = (float)z;
             for (int e = tid; e < to; e += 256) {
                 float sz, m2d, m4d, m5, m3;
                 if (e >= from) {
@@ -1232,10 +1235,13 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 } else {
                     const float4 zz = s_z[e];
                     sz = zz.x; m2d = zz.y; m4d = zz.z; m5 = zz.w;
-                    m3 = s_u[e].w;
+                    m3 = s_u[0][e].w;
                 }
-                const float dz = fz - sz;
-                s_u[e] = make_float4(dz * m2d, dz * m4d, (dz * dz) * m5, m3);
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const float dz = (float)(z + k * K) - sz;
+                    s_u[k][e] = make_float4(dz * m2d, dz * m4d, (dz * dz) * m5, m3);
+                }
             }
         };
         auto slab_mask = [&](int slot) {
@@ -1259,11 +1265,19 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             }
         };
 
-        MaskT m_lo = 0, m_mid = 0, m_hi = 0;  // slabs z-K, z, z+K
-        int ids_new[NRS];
+        // masks of slabs z - K, z, ..., z + S K at this thread's column;
+        // slices z, z + K, ... (S of them) are evaluated per step
+        MaskT m[S + 2];
+#pragma unroll
+        for (int j = 0; j < S + 2; ++j) m[j] = 0;
+        int ids_new[S][NRS];
         int z = r + K * g * c.seg;
-        uint32_t pk_next = 0u;
-        if (kLabAhead && mine) pk_next = g_lab[((size_t)z * H + y) * W + x];
+        uint32_t pk_next[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            pk_next[k] = 0u;
+            if (kLabAhead && mine && k < n) pk_next[k] = g_lab[((size_t)(z + k * K) * H + y) * W + x];
+        }
         bool full = true;
         for (int step = 0; step < n;) {
             bool overflow = false;
@@ -1271,106 +1285,118 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 release();
                 __syncthreads();
                 if (tid == 0) s_count = 0;
-                int ids0[NRS], ids1[NRS], ids2[NRS];
-                load_slab(clampi(z - K, 0, D - 1), ids0);
-                load_slab(z, ids1);
-                load_slab(clampi(z + K, 0, D - 1), ids2);
+                int ids_all[S + 2][NRS];
+#pragma unroll
+                for (int j = 0; j < S + 2; ++j) load_slab(clampi(z + (j - 1) * K, 0, D - 1), ids_all[j]);
                 __syncthreads();
-                claim(ids0);
-                claim(ids1);
-                claim(ids2);
+#pragma unroll
+                for (int j = 0; j < S + 2; ++j) claim(ids_all[j]);
                 __syncthreads();
                 known = s_count;
                 overflow = known > MC;
                 if (!overflow) {
-                    convert(ids0, 0);
-                    convert(ids1, 1);
-                    convert(ids2, 2);
+#pragma unroll
+                    for (int j = 0; j < S + 2; ++j) convert(ids_all[j], j);
                     stage(0, known, z);
                 }
                 __syncthreads();
                 if (!overflow) {
-                    m_lo = slab_mask(0);
-                    m_mid = slab_mask(1);
-                    m_hi = slab_mask(2);
+#pragma unroll
+                    for (int j = 0; j < S + 2; ++j) m[j] = slab_mask(j);
                 }
-                full = overflow;              // rebuild again at the next slice
+                full = overflow;              // rebuild again at the next step
             } else {
-                claim(ids_new);
+#pragma unroll
+                for (int k = 0; k < S; ++k) claim(ids_new[k]);
                 __syncthreads();
                 const int cnt = s_count;
-                if (cnt > MC) {               // uniform: rebuild this slice
+                if (cnt > MC) {               // uniform: rebuild this step
                     known = cnt;
                     full = true;
                     continue;
                 }
-                convert(ids_new, 0);
+#pragma unroll
+                for (int k = 0; k < S; ++k) convert(ids_new[k], k);
                 stage(known, cnt, z);
                 known = cnt;
                 __syncthreads();
-                m_lo = m_mid;
-                m_mid = m_hi;
-                m_hi = slab_mask(0);
+                m[0] = m[S];
+                m[1] = m[S + 1];
+#pragma unroll
+                for (int k = 0; k < S; ++k) m[2 + k] = slab_mask(k);
             }
-            // the next step's new slab (and LAB), loaded while this slice is evaluated
-            const uint32_t pk_here = pk_next;
-            if (step + 1 < n) {
-                load_slab(clampi(z + 2 * K, 0, D - 1), ids_new);
-                if (kLabAhead && mine) pk_next = g_lab[((size_t)(z + K) * H + y) * W + x];
+            const int ns = min(S, n - step);  // slices of this step
+            // the next step's new slabs (and LAB), loaded while this step is evaluated
+            uint32_t pk_here[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) pk_here[k] = pk_next[k];
+            if (step + S < n) {
+                const int zn = z + S * K;
+#pragma unroll
+                for (int k = 0; k < S; ++k) load_slab(clampi(zn + (k + 1) * K, 0, D - 1), ids_new[k]);
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    if (kLabAhead && mine && step + S + k < n) pk_next[k] = g_lab[((size_t)(zn + k * K) * H + y) * W + x];
             }
 
             if (mine) {
-                const size_t v = ((size_t)z * H + y) * W + x;
-                const uint32_t pk = kLabAhead ? pk_here : g_lab[v];
-                const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u), l2 = (float)((pk >> 16) & 255u);
-                unsigned long long best = ASV_KEY_INIT;
-                if (!overflow) {
-                    MaskT mask = (m_lo | m_mid | m_hi) & ~kBadBit;
-                    while (mask) {
-                        const int e = MC == 31 ? __builtin_ctz((uint32_t)mask) : __builtin_ctzll((unsigned long long)mask);
-                        mask &= mask - 1;
-                        const float4 p = s_p[e], u = s_u[e], cc = s_c[e];
-                        const float dx = fx - p.x, dy = fy - p.y;
-                        const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
-                        const float t2 = __fmaf_rn(dy, u.w, u.y);
-                        const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
-                        const float c0 = l0 - cc.x, c1 = l1 - cc.y, c2 = l2 - cc.z;
-                        const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
-                        const unsigned long long kk =
-                            ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.w);
-                        best = kk < best ? kk : best;
-                    }
-                } else {
-                    const float fz = (float)z;
-                    const int xs3[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
-                    const int ys3[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
-                    const int zs3[3] = {z, clampi(z - K, 0, D - 1), clampi(z + K, 0, D - 1)};
-                    int seen0 = -1, seen1 = -1;
-#pragma unroll 1
-                    for (int sl = 0; sl < 3; ++sl)
-#pragma unroll 1
-                        for (int rr = 0; rr < 3; ++rr) {
-                            const gptr<const int32_t> row = g_in + ((size_t)zs3[sl] * H + ys3[rr]) * W;
-#pragma unroll 1
-                            for (int cc = 0; cc < 3; ++cc) {
-                                const int id = row[xs3[cc]];
-                                if (id < 0 || id == seen0 || id == seen1) continue;
-                                seen1 = seen0;
-                                seen0 = id;
-                                const gptr<const float> q = g_rec + (size_t)id * 12;
-                                const float d = cand_dist(ld_f4(q), ld_f4(q + 4), ld_f4(q + 8), fx, fy, fz, l0, l1,
-                                                          l2, ratio);
-                                const unsigned long long kk = dkey(d, id);
-                                best = kk < best ? kk : best;
-                            }
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    if (k >= ns) break;
+                    const int zk = z + k * K;
+                    const size_t v = ((size_t)zk * H + y) * W + x;
+                    const uint32_t pk = kLabAhead ? pk_here[k] : g_lab[v];
+                    const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u),
+                                l2 = (float)((pk >> 16) & 255u);
+                    unsigned long long best = ASV_KEY_INIT;
+                    if (!overflow) {
+                        MaskT mask = (m[k] | m[k + 1] | m[k + 2]) & ~kBadBit;
+                        while (mask) {
+                            const int e = MC == 31 ? __builtin_ctz((uint32_t)mask)
+                                                   : __builtin_ctzll((unsigned long long)mask);
+                            mask &= mask - 1;
+                            const float4 p = s_p[e], u = s_u[k][e], cc = s_c[e];
+                            const float dx = fx - p.x, dy = fy - p.y;
+                            const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
+                            const float t2 = __fmaf_rn(dy, u.w, u.y);
+                            const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
+                            const float c0 = l0 - cc.x, c1 = l1 - cc.y, c2 = l2 - cc.z;
+                            const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
+                            const unsigned long long kk =
+                                ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.w);
+                            best = kk < best ? kk : best;
                         }
+                    } else {
+                        const float fz = (float)zk;
+                        const int xs3[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
+                        const int ys3[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
+                        const int zs3[3] = {zk, clampi(zk - K, 0, D - 1), clampi(zk + K, 0, D - 1)};
+                        int seen0 = -1, seen1 = -1;
+#pragma unroll 1
+                        for (int sl = 0; sl < 3; ++sl)
+#pragma unroll 1
+                            for (int rr = 0; rr < 3; ++rr) {
+                                const gptr<const int32_t> row = g_in + ((size_t)zs3[sl] * H + ys3[rr]) * W;
+#pragma unroll 1
+                                for (int cc = 0; cc < 3; ++cc) {
+                                    const int id = row[xs3[cc]];
+                                    if (id < 0 || id == seen0 || id == seen1) continue;
+                                    seen1 = seen0;
+                                    seen0 = id;
+                                    const gptr<const float> q = g_rec + (size_t)id * 12;
+                                    const float d = cand_dist(ld_f4(q), ld_f4(q + 4), ld_f4(q + 8), fx, fy, fz, l0,
+                                                              l1, l2, ratio);
+                                    const unsigned long long kk = dkey(d, id);
+                                    best = kk < best ? kk : best;
+                                }
+                            }
+                    }
+                    g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
                 }
-                g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
             }
-            z += K;
-            ++step;
+            z += S * K;
+            step += S;
         }
-
     }
 }
 
@@ -1647,23 +1673,42 @@ int march_mc(int k)
     return k <= 3 ? 31 : 63;
 }
 
+// Slices of a chain evaluated per step of the marching pass at step k: two
+// halve the barriers, table round trips and step bookkeeping per voxel and
+// need a fourth slab mask per thread.  The environment variable ASV_MARCH_S
+// (1 or 2) overrides the choice for every step.
+int march_s(int k)
+{
+    static int forced = -1;
+    if (forced < 0) {
+        const char* e = getenv("ASV_MARCH_S");
+        const int v = e ? atoi(e) : 0;
+        forced = (v == 1 || v == 2) ? v : 0;
+    }
+    if (forced) return forced;
+    // measured at 640x480x200: K = 1 593 vs 608 us, K = 2 420 vs 383,
+    // K = 3 622 vs 566, K = 5 573 vs 511 (one vs two slices per step)
+    return k >= 2 ? 2 : 1;
+}
+
 using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
 
 template <int K>
-static MarchFn march_fn(int mc)
+static MarchFn march_fn(int mc, int sl)
 {
-    return mc == 31 ? k_jfa_march<K, 31> : k_jfa_march<K, 63>;
+    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2> : k_jfa_march<K, 63, 2>;
+    return mc == 31 ? k_jfa_march<K, 31, 1> : k_jfa_march<K, 63, 1>;
 }
 
-static MarchFn march_kernel(int k, int mc)
+static MarchFn march_kernel(int k, int mc, int sl)
 {
     switch (k) {
-    case 1: return march_fn<1>(mc);   case 2: return march_fn<2>(mc);   case 3: return march_fn<3>(mc);
-    case 4: return march_fn<4>(mc);   case 5: return march_fn<5>(mc);   case 6: return march_fn<6>(mc);
-    case 7: return march_fn<7>(mc);   case 8: return march_fn<8>(mc);   case 9: return march_fn<9>(mc);
-    case 10: return march_fn<10>(mc); case 11: return march_fn<11>(mc); case 12: return march_fn<12>(mc);
-    case 13: return march_fn<13>(mc); case 14: return march_fn<14>(mc); case 15: return march_fn<15>(mc);
-    case 16: return march_fn<16>(mc); default: return nullptr;
+    case 1: return march_fn<1>(mc, sl);   case 2: return march_fn<2>(mc, sl);   case 3: return march_fn<3>(mc, sl);
+    case 4: return march_fn<4>(mc, sl);   case 5: return march_fn<5>(mc, sl);   case 6: return march_fn<6>(mc, sl);
+    case 7: return march_fn<7>(mc, sl);   case 8: return march_fn<8>(mc, sl);   case 9: return march_fn<9>(mc, sl);
+    case 10: return march_fn<10>(mc, sl); case 11: return march_fn<11>(mc, sl); case 12: return march_fn<12>(mc, sl);
+    case 13: return march_fn<13>(mc, sl); case 14: return march_fn<14>(mc, sl); case 15: return march_fn<15>(mc, sl);
+    case 16: return march_fn<16>(mc, sl); default: return nullptr;
     }
 }
 
@@ -1679,16 +1724,16 @@ static int march_dyn_bytes(int amax)
 // 8), cached per step, capacity and table size.
 int march_max_grid(int k, int dyn)
 {
-    static int cache[17][2][kMarchMaxIds / 1024 + 1];
+    static int cache[17][2][2][kMarchMaxIds / 1024 + 1];
     if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchMaxIds) return 0;
-    const int mc = march_mc(k);
-    int& slot = cache[k][mc == 63][dyn / 1024];
+    const int mc = march_mc(k), sl = march_s(k);
+    int& slot = cache[k][mc == 63][sl == 2][dyn / 1024];
     int g = __atomic_load_n(&slot, __ATOMIC_RELAXED);
     if (g == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc), 256,
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl), 256,
                                                            (size_t)dyn);
         g = cus * (per_cu > 0 ? per_cu : 1);
         g = (g / 8) * 8;
@@ -1770,7 +1815,7 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
 hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
                         int dyn)
 {
-    MarchFn fn = march_kernel(k, march_mc(k));
+    MarchFn fn = march_kernel(k, march_mc(k), march_s(k));
     if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
     JfaClip a0 = one;
     const JfaClip* a1 = dev;
