@@ -5,9 +5,10 @@ The stage methods mirror the reference's Image interface
 (GPU-AS/Render.cpp:254-319):
 
     init_gpu           -> Clip(params)                  (asv_create)
-    RGB2LAB + metric   -> Clip.prepare()                (asv_prepare)
+    RGB2LAB            -> Clip.prepare()                (asv_prepare)
     Ld_first_init /
-    Ld_next_seeds_matrix -> Clip.seed_stage(it)         (asv_seed_stage)
+    Ld_next_seeds_matrix -> Clip.seed_stage(it)         (asv_seed_stage;
+                          matrix_slover evaluated at the seeds and rings)
     flood              -> Clip.flood(it)                (asv_flood)
     centroids_next_seeds -> Clip.centroids(it)          (asv_centroids)
     label readback     -> Clip.labels()                 (asv_download_labels)
@@ -297,6 +298,8 @@ class Clip:
         return out
 
     def metric(self):
+        """The per-voxel metric (m0..m5, psd flag, 0) of every voxel,
+        computed on request (the Lloyd loop evaluates it only at seeds)."""
         out = np.empty(self.shape + (8,), np.float32)
         check(self.lib.asv_download_metric(self.h, ptr(out), self.stream))
         return out

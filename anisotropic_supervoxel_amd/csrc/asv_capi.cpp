@@ -148,7 +148,7 @@ namespace {
 
 void free_ctx(asv_ctx* c)
 {
-    void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->field, c->fix_list, c->fix_count,
+    void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->seed_own, c->seed_need,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
                     c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys};
     for (void* q : ptrs)
@@ -171,10 +171,8 @@ int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace, bool keys)
     b += V * 3;                  // rgb
     b += V * 16;                 // fwd + bwd
     b += V * 4;                  // lab
-    b += V * 32;                 // metric field
-    b += V * 4;                  // fixup list
     b += V * 8;                  // labels x2
-    b += A * (16 * 3 + 48);      // grid, pos, slab, rec
+    b += A * (16 * 3 + 48 + 36); // grid, pos, slab, rec, seed_own, seed_need
     b += (A + 1) * 64 * 2;       // sums x2
     if (trace) b += (int64_t)iters * A * 48;
     return b + 4096;
@@ -279,15 +277,14 @@ int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out
     if (e == hipSuccess) e = dalloc(&c->bwd, (size_t)V, &tb);
     if (e == hipSuccess) e = dalloc(&c->lut, 256, &tb);
     if (e == hipSuccess) e = dalloc(&c->lab, (size_t)V, &tb);
-    if (e == hipSuccess) e = dalloc(&c->field, (size_t)V * 2, &tb);
-    if (e == hipSuccess) e = dalloc(&c->fix_list, (size_t)V, &tb);
-    if (e == hipSuccess) e = dalloc(&c->fix_count, 4, &tb);
     if (e == hipSuccess) e = dalloc(&c->labels[0], (size_t)V, &tb);
     if (e == hipSuccess) e = dalloc(&c->labels[1], (size_t)V, &tb);
     if (e == hipSuccess) e = dalloc(&c->grid, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->pos, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->slab, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->rec, (size_t)Acap, &tb);
+    if (e == hipSuccess) e = dalloc(&c->seed_own, (size_t)Acap * 2, &tb);
+    if (e == hipSuccess) e = dalloc(&c->seed_need, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(Acap + 1) * kSumWords, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(Acap + 1) * kSumWords, &tb);
     if (e == hipSuccess && wants_keys(p, c->steps, A)) {
@@ -570,10 +567,7 @@ int asv_prepare(asv_ctx* c, void* stream)
     int r = check_ready(c);
     if (r) return r;
     hipStream_t st = S(stream);
-    HIP_TRY(launch_zero_i32(st, c->fix_count, 1));
-    HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->fwd, c->bwd, c->lut, c->lab, c->field,
-                           c->fix_list, c->fix_count));
-    HIP_TRY(launch_fixup(st, c->d, c->fwd, c->bwd, c->field, c->fix_list, c->fix_count));
+    HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->lut, c->lab));
     if (c->p.seed_init == ASV_INIT_DENSITY && c->gdims[0] > 0 && (r = density_reseed(c, st))) return r;
     c->prepared = true;
     return ASV_OK;
@@ -596,7 +590,8 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
     c->cur = 0;
     float* tr = c->trace_on && c->trace ? c->trace + (size_t)(iter - 1) * c->d.A * 12 : nullptr;
     int gate = (iter >= 2 && c->p.angle_gate) ? 1 : 0;
-    HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->field, c->fwd, gate, c->rec, tr));
+    HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->fwd, c->bwd, gate, c->rec, c->seed_own, c->seed_need,
+                               tr));
     HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
     HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
     c->fresh = true;
@@ -1009,8 +1004,15 @@ int asv_download_metric(asv_ctx* c, float* m8, void* stream)
 {
     if (!c || !m8) return fail(ASV_ERR_ARG, "null argument");
     if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run");
-    HIP_TRY(hipMemcpyAsync(m8, c->field, (size_t)c->V() * 32, hipMemcpyDefault, S(stream)));
-    HIP_TRY(hipStreamSynchronize(S(stream)));
+    hipStream_t st = S(stream);
+    float4* field = nullptr;
+    HIP_TRY(hipMalloc(&field, (size_t)c->V() * 32));
+    hipError_t e = launch_metric_field(st, c->d, c->fwd, c->bwd, field);
+    if (e == hipSuccess) e = hipMemcpyAsync(m8, field, (size_t)c->V() * 32, hipMemcpyDefault, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t ef = hipFree(field);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) return fail(ASV_ERR_HIP, "metric field: %s", hipGetErrorString(e));
     return ASV_OK;
 }
 

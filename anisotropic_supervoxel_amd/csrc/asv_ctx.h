@@ -30,14 +30,13 @@ struct asv_ctx {
     float2* bwd = nullptr;
     float* lut = nullptr;
     uint32_t* lab = nullptr;
-    float4* field = nullptr;
-    int* fix_list = nullptr;
-    int* fix_count = nullptr;
     int32_t* labels[2] = {nullptr, nullptr};
     int4* grid = nullptr;
     int4* pos = nullptr;
     float4* slab = nullptr;
     asv::SeedRec* rec = nullptr;
+    float4* seed_own = nullptr;     // a seed's own (non-PSD) metric for the ring fallback
+    int* seed_need = nullptr;       // 1 where the ring fallback runs
     unsigned long long* sums[2] = {nullptr, nullptr};
     float* trace = nullptr;
     int32_t* map_table = nullptr;

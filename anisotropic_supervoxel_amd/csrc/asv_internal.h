@@ -41,11 +41,11 @@ struct Dims {
 };
 
 // Launchers (all asynchronous on `st`).  Return hipSuccess or the launch error.
-hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared,
-                          const float2* fwd, const float2* bwd, const float* lin_lut,
-                          uint32_t* lab, float4* field, int* fix_list, int* fix_count);
-hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd,
-                        float4* field, const int* fix_list, const int* fix_count);
+hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared, const float* lin_lut,
+                          uint32_t* lab);
+// The metric of every voxel into field (2 float4 per voxel: m0..m3, m4, m5,
+// psd flag, 0), for tests and tools.
+hipError_t launch_metric_field(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd, float4* field);
 // Colour-gradient density of every grid seed's box (GPU-AS/Image.cpp:350-560):
 // acc[slot] += 4096 * g over the voxels whose float16 grid id addresses
 // slot (< nslot); mx / my / mz map x / y / z to the grid index along each
@@ -54,9 +54,11 @@ hipError_t launch_density(hipStream_t st, const Dims& d, const uint8_t* rgb, int
                           const int* my, const int* mz, int wnum, int hnum, int nslot, unsigned long long* acc);
 hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, const uint32_t* lab,
                             int4* pos, float4* slab);
-hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab,
-                              const float4* field, const float2* fwd, int gate, SeedRec* rec,
-                              float* trace_iter);
+// Seed metrics and records: the metric at each seed's voxel, ring fallbacks
+// for the seeds where it is not PSD (own: 2 float4 per seed, need: 1 int
+// per seed, scratch of the launch).
+hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab, const float2* fwd,
+                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter);
 hipError_t launch_fill(hipStream_t st, int32_t* p, int64_t n, int32_t value);
 hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n);
 hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);

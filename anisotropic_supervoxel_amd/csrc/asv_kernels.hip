@@ -8,12 +8,13 @@
 // is no MFMA work here; the bound is HBM bandwidth.
 //
 // Reference passes replaced (paths relative to the reference root):
-//   k_prepare / k_fixup   shaders/RGB2LAB.fs:11-79 + Image.cpp:874-1009
+//   k_prepare             shaders/RGB2LAB.fs:11-79 (blur + flip of Render.cpp:354-359)
 //   k_seed_init           shaders/draw_seeds.vs:20-35 (colour at the seed voxel)
-//   k_seed_metric         Image.cpp:689-872 and :1351-1534
+//   k_seed_metric(_rings) Image.cpp:689-872 and :1351-1534, with matrix_slover
+//                         (Image.cpp:874-1009) evaluated at the seeds and rings
 //   k_fill + k_place      draw_seeds / draw_next_seeds (clear -1, later id wins)
 //   k_jfa                 shaders/flood.gs:53-209
-//   k_reduce_bricks       shaders/centroids.vs:19-53 with additive blending
+//   k_reduce_rows         shaders/centroids.vs:19-53 with additive blending
 //   k_finalize            shaders/aver_exception.vs:22-81
 
 #include "asv_internal.h"
@@ -52,21 +53,35 @@ __device__ __forceinline__ int reflect101(int i, int n)
 // LAB (RGB2LAB.fs:11-79)
 // ---------------------------------------------------------------------------
 
-// Cube root on (0, 1.1] by five Halley steps in double from a linear start.
-__device__ __forceinline__ double cbrt_halley(double x)
+// Cube root of the white-normalised X, Y or Z: the same float as the
+// specification's five Halley steps in double from 0.45 + 0.55 r, rounded to
+// float, for every ratio the LAB kernel meets, at a fifth of the double
+// work: an inverse cube root z ~ r^(-1/3)
+// from the exponent bits, two Newton steps z += z (1 - r z^3) / 3 in float
+// and two in double, then r z^2 rounded to float.  Checked equal, bit for
+// bit, on all 3 x 2^24 ratios of the 2^24 RGB inputs (and the LAB volume of
+// every RGB triple is compared with the oracle's).
+__device__ __forceinline__ float cbrt_fast(float r)
 {
-    double y = 0.45 + 0.55 * x;
+    float z = __uint_as_float(0x54a2fa8cu - __float_as_uint(r) / 3u);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double y3 = y * y * y;
-        y = y * ((y3 + 2.0 * x) / (2.0 * y3 + x));
+    for (int i = 0; i < 2; ++i) {
+        const float z3 = z * z * z;
+        z = z + z * ((1.0f - r * z3) * 0.333333343f);
     }
-    return y;
+    double zd = z;
+    const double rd = r;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double z3 = zd * zd * zd;
+        zd = zd + zd * ((1.0 - rd * z3) * (1.0 / 3.0));
+    }
+    return (float)(rd * zd * zd);
 }
 
 __device__ __forceinline__ float lab_f(float r)
 {
-    if (r > 0.008856f) return (float)cbrt_halley((double)r);
+    if (r > 0.008856f) return cbrt_fast(r);
     return (903.3f * r + 16.0f) / 116.0f;
 }
 
@@ -113,16 +128,29 @@ __device__ __forceinline__ int normal_equations(const float2* __restrict__ fwd, 
     for (int i = 0; i < 6; ++i) c[i] = 0.0;
     int rows = 0;
     const size_t frame = (size_t)t * H * W;
+    // every flow tap is loaded before the first row is accumulated, so the
+    // 18 loads are in flight together
+    float2 ff[9], fb[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int nx = x + k % 3 - 1, ny = y + k / 3 - 1;
+        const bool in = nx >= 0 && nx < W && ny >= 0 && ny < H;
+        const size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;   // image row of GL row ny
+        ff[k] = in && t != D - 1 ? fwd[pix] : make_float2(0.0f, 0.0f);
+        fb[k] = in && t != 0 ? bwd[pix] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
     for (int m = -1; m <= 1; ++m) {
+#pragma unroll
         for (int n = -1; n <= 1; ++n) {
             int nx = x + n, ny = y + m;
             if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
             double b = (double)c_expected_b[(m + 1) * 3 + (n + 1)];
-            size_t pix = frame + (size_t)(H - 1 - ny) * W + nx;   // image row of GL row ny
+#pragma unroll
             for (int dir = 0; dir < 2; ++dir) {
                 if (dir == 0 && t == D - 1) continue;
                 if (dir == 1 && t == 0) continue;
-                float2 f = dir == 0 ? fwd[pix] : bwd[pix];
+                float2 f = dir == 0 ? ff[(m + 1) * 3 + (n + 1)] : fb[(m + 1) * 3 + (n + 1)];
                 float fu = f.x, fv = -f.y;
                 float dx = (float)n + floorf(fu + 0.5f);
                 float dy = (float)m + floorf(fv + 0.5f);
@@ -251,14 +279,50 @@ __device__ __forceinline__ int psd3(float m0, float m1, float m2, float m3, floa
     return 1;
 }
 
-__device__ __forceinline__ void store_field(float4* field, size_t v, const double m[6])
+// Minimum-norm solution of a rank-deficient system through the eigen-
+// decomposition of N (pseudo-inverse, eigenvalues below 1e-9 of the largest
+// dropped).  Kept out of line: only voxels whose LDL^T pivot collapses take
+// it, and its 6 x 6 work arrays would otherwise weigh on every caller's
+// registers.
+__device__ __noinline__ void pinv_solve(const double* N, const double* c, double* m)
 {
-    float f[6];
+    double a[36], ev[36];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) a[i * 6 + j] = N[tri(i, j)];
+    jacobi_sym<6>(a, ev, 30);
+    double lmax = 0.0;
+    for (int i = 0; i < 6; ++i) lmax = a[i * 6 + i] > lmax ? a[i * 6 + i] : lmax;
+    for (int k = 0; k < 6; ++k) m[k] = 0.0;
+    for (int e = 0; e < 6; ++e) {
+        double lam = a[e * 6 + e];
+        if (!(lam > 1e-9 * lmax)) continue;
+        double proj = 0.0;
+        for (int k = 0; k < 6; ++k) proj = proj + ev[k * 6 + e] * c[k];
+        double coef = proj / lam;
+        for (int k = 0; k < 6; ++k) m[k] = m[k] + ev[k * 6 + e] * coef;
+    }
+}
+
+// The metric of voxel (x, y, t) (matrix_slover, GPU-AS/Image.cpp:874-1009):
+// the least-squares 6-vector rounded to float into m, and whether the 3 x 3
+// matrix it spells is positive semi-definite.  No rows (a 1 x 1 frame of a
+// one-frame clip) gives the zero matrix.  The reference evaluates it only at
+// the seeds and their rings, every iteration; so does this build (the seed
+// kernels call it), instead of keeping a field for every voxel.
+__device__ __forceinline__ int voxel_metric(const float2* __restrict__ fwd, const float2* __restrict__ bwd, int W,
+                                            int H, int D, int x, int y, int t, float m[6])
+{
+    double N[21], c[6], md[6];
+    const int rows = normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
+    if (rows == 0) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) f[i] = (float)m[i];
-    int p = psd3(f[0], f[1], f[2], f[3], f[4], f[5]);
-    field[2 * v] = make_float4(f[0], f[1], f[2], f[3]);
-    field[2 * v + 1] = make_float4(f[4], f[5], p ? 1.0f : 0.0f, 0.0f);
+        for (int i = 0; i < 6; ++i) md[i] = 0.0;
+    } else if (!ldl_solve(N, c, md)) {
+        pinv_solve(N, c, md);
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m[i] = (float)md[i];
+    return psd3(m[0], m[1], m[2], m[3], m[4], m[5]);
 }
 
 // (R, G, B) of voxel (x, y, t), GL row order, as the reference's frame
@@ -291,13 +355,10 @@ __device__ __forceinline__ uint3 frame_rgb(const uint8_t* __restrict__ rgb, int 
     return make_uint3((ar + 8u) >> 4, (ag + 8u) >> 4, (ab + 8u) >> 4);
 }
 
-// Kernel (a): blur + flip + LAB + metric for one voxel per thread.  A voxel
-// whose normal equations are rank deficient is queued for k_fixup.
+// Kernel (a): blur + flip + LAB for one voxel per thread.
 __global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint8_t* __restrict__ rgb,
-                                                 int prepared, const float2* __restrict__ fwd,
-                                                 const float2* __restrict__ bwd, const float* __restrict__ lin_lut,
-                                                 uint32_t* __restrict__ lab, float4* __restrict__ field,
-                                                 int* __restrict__ fix_list, int* __restrict__ fix_count)
+                                                 int prepared, const float* __restrict__ lin_lut,
+                                                 uint32_t* __restrict__ lab)
 {
     __shared__ float lin[256];
     int tid = threadIdx.y * 64 + threadIdx.x;
@@ -309,58 +370,23 @@ __global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint
     int t = blockIdx.z;
     if (x >= W || y >= H) return;
     size_t v = ((size_t)t * H + y) * W + x;
-
     const uint3 c3 = frame_rgb(rgb, prepared, W, H, x, y, t);
-    const uint32_t cr = c3.x, cg = c3.y, cb = c3.z;
-    lab[v] = lab_pack(cr, cg, cb, lin);
-
-    double N[21], c[6], m[6];
-    int rows = normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
-    if (rows == 0) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) m[i] = 0.0;
-        store_field(field, v, m);
-        return;
-    }
-    if (ldl_solve(N, c, m)) {
-        store_field(field, v, m);
-    } else {
-        int slot = atomicAdd(fix_count, 1);
-        fix_list[slot] = (int)v;
-    }
+    lab[v] = lab_pack(c3.x, c3.y, c3.z, lin);
 }
 
-// Rank-deficient voxels: minimum-norm solution through the eigen-
-// decomposition of N (pseudo-inverse, eigenvalues below 1e-9 of the largest
-// dropped).
-__global__ __launch_bounds__(64) void k_fixup(int W, int H, int D, const float2* __restrict__ fwd,
-                                              const float2* __restrict__ bwd, float4* __restrict__ field,
-                                              const int* __restrict__ fix_list, const int* __restrict__ fix_count)
+// The metric of every voxel (m0..m5, psd flag, 0 per voxel), for tests and
+// tools; the Lloyd loop itself evaluates the metric only where the seed
+// kernels need it.
+__global__ __launch_bounds__(256) void k_metric_field(int W, int H, int D, const float2* __restrict__ fwd,
+                                                      const float2* __restrict__ bwd, float4* __restrict__ field)
 {
-    int n = *fix_count;
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        int v = fix_list[idx];
-        int x = v % W;
-        int y = (v / W) % H;
-        int t = v / (W * H);
-        double N[21], c[6];
-        normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
-        double a[36], ev[36];
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) a[i * 6 + j] = N[tri(i, j)];
-        jacobi_sym<6>(a, ev, 30);
-        double lmax = 0.0;
-        for (int i = 0; i < 6; ++i) lmax = a[i * 6 + i] > lmax ? a[i * 6 + i] : lmax;
-        double m[6] = {0, 0, 0, 0, 0, 0};
-        for (int e = 0; e < 6; ++e) {
-            double lam = a[e * 6 + e];
-            if (!(lam > 1e-9 * lmax)) continue;
-            double proj = 0.0;
-            for (int k = 0; k < 6; ++k) proj = proj + ev[k * 6 + e] * c[k];
-            double coef = proj / lam;
-            for (int k = 0; k < 6; ++k) m[k] = m[k] + ev[k * 6 + e] * coef;
-        }
-        store_field(field, (size_t)v, m);
+    const int64_t V = (int64_t)W * H * D;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(v % W), y = (int)((v / W) % H), t = (int)(v / ((int64_t)W * H));
+        float m[6];
+        const int p = voxel_metric(fwd, bwd, W, H, D, x, y, t, m);
+        field[2 * v] = make_float4(m[0], m[1], m[2], m[3]);
+        field[2 * v + 1] = make_float4(m[4], m[5], p ? 1.0f : 0.0f, 0.0f);
     }
 }
 
@@ -486,54 +512,9 @@ __device__ void nearest_psd(const float in[6], float out[6])
     out[3] = (float)r[4]; out[4] = (float)r[5]; out[5] = (float)r[8];
 }
 
-// Metric of every seed: the field at its voxel if PSD, else the mean of the
-// PSD 1-ring (then 2-ring) neighbours on the same frame, else the nearest
-// PSD matrix; the ring neighbours pass the flow-angle gate when `gate`.
-__global__ void k_seed_metric(int W, int H, int D, int A, const int4* __restrict__ pos,
-                              const float4* __restrict__ slab, const float4* __restrict__ field,
-                              const float2* __restrict__ fwd, int gate, SeedRec* __restrict__ rec,
-                              float* __restrict__ trace_iter)
+__device__ __forceinline__ void write_seed(int i, int4 p, float4 c, const float m[6], SeedRec* __restrict__ rec,
+                                           float* __restrict__ trace_iter)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A) return;
-    int4 p = pos[i];
-    size_t v = ((size_t)p.z * H + p.y) * W + p.x;
-    float4 f0 = field[2 * v], f1 = field[2 * v + 1];
-    float m[6];
-    if (f1.z != 0.0f) {
-        m[0] = f0.x; m[1] = f0.y; m[2] = f0.z; m[3] = f0.w; m[4] = f1.x; m[5] = f1.y;
-    } else {
-        float2 sf = make_float2(0.0f, 0.0f);
-        if (gate && p.z != D - 1) sf = flow_gl(fwd, W, H, p.x, p.y, p.z);
-        float sum[6] = {0, 0, 0, 0, 0, 0};
-        int cnt = 0;
-        for (int ring = 0; ring < 2 && cnt == 0; ++ring) {
-            int nr = ring == 0 ? 8 : 16;
-            for (int t = 0; t < nr; ++t) {
-                int nx = p.x + (ring == 0 ? c_ring1x[t] : c_ring2x[t]);
-                int ny = p.y + (ring == 0 ? c_ring1y[t] : c_ring2y[t]);
-                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                if (gate) {
-                    float2 nf = make_float2(0.0f, 0.0f);
-                    if (p.z != D - 1) nf = flow_gl(fwd, W, H, nx, ny, p.z);
-                    if (!angle_gate(sf.x, sf.y, nf.x, nf.y)) continue;
-                }
-                size_t nvx = ((size_t)p.z * H + ny) * W + nx;
-                float4 g0 = field[2 * nvx], g1 = field[2 * nvx + 1];
-                if (g1.z == 0.0f) continue;
-                sum[0] = sum[0] + g0.x; sum[1] = sum[1] + g0.y; sum[2] = sum[2] + g0.z;
-                sum[3] = sum[3] + g0.w; sum[4] = sum[4] + g1.x; sum[5] = sum[5] + g1.y;
-                ++cnt;
-            }
-        }
-        if (cnt > 0) {
-            for (int k = 0; k < 6; ++k) m[k] = sum[k] / (float)cnt;
-        } else {
-            float self[6] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y};
-            nearest_psd(self, m);
-        }
-    }
-    float4 c = slab[i];
     SeedRec r;
     r.a = make_float4((float)p.x, (float)p.y, (float)p.z, m[0]);
     r.b = make_float4(2.0f * m[1], 2.0f * m[2], m[3], 2.0f * m[4]);
@@ -545,6 +526,91 @@ __global__ void k_seed_metric(int W, int H, int D, int A, const int4* __restrict
         tr[3] = c.x; tr[4] = c.y; tr[5] = c.z;
         for (int k = 0; k < 6; ++k) tr[6 + k] = m[k];
     }
+}
+
+// Metric of every seed, first part: the metric at the seed's voxel.  A PSD
+// one is the seed's metric and its record is written here; otherwise the
+// voxel's metric is parked in `own` (two float4 per seed) and `need` asks
+// k_seed_metric_rings for the ring fallbacks.
+__global__ __launch_bounds__(64) void k_seed_metric(int W, int H, int D, int A, const int4* __restrict__ pos,
+                                                    const float4* __restrict__ slab, const float2* __restrict__ fwd,
+                                                    const float2* __restrict__ bwd, SeedRec* __restrict__ rec,
+                                                    float4* __restrict__ own, int* __restrict__ need,
+                                                    float* __restrict__ trace_iter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A) return;
+    const int4 p = pos[i];
+    float m[6];
+    const int ok = voxel_metric(fwd, bwd, W, H, D, p.x, p.y, p.z, m);
+    need[i] = ok ? 0 : 1;
+    if (ok) {
+        write_seed(i, p, slab[i], m, rec, trace_iter);
+    } else {
+        own[2 * i] = make_float4(m[0], m[1], m[2], m[3]);
+        own[2 * i + 1] = make_float4(m[4], m[5], 0.0f, 0.0f);
+    }
+}
+
+// Second part, one wave per seed, only for seeds whose own metric is not
+// PSD: lanes 0..7 evaluate the 1-ring and lanes 8..23 the 2-ring on the
+// seed's frame together (in bounds and, when `gate`, through the flow-angle
+// gate); lane 0 then takes the mean of the PSD 1-ring metrics, else of the
+// PSD 2-ring ones, each summed in ring order, else the nearest PSD matrix
+// of the seed's own metric.
+__global__ __launch_bounds__(64) void k_seed_metric_rings(int W, int H, int D, const int4* __restrict__ pos,
+                                                          const float4* __restrict__ slab,
+                                                          const float2* __restrict__ fwd,
+                                                          const float2* __restrict__ bwd, int gate,
+                                                          SeedRec* __restrict__ rec, const float4* __restrict__ own,
+                                                          const int* __restrict__ need, float* __restrict__ trace_iter)
+{
+    const int i = blockIdx.x;
+    if (!need[i]) return;
+    __shared__ float s_m[24][6];
+    __shared__ int s_ok[24];
+    const int lane = threadIdx.x;
+    const int4 p = pos[i];
+    if (lane < 24) {
+        const int nx = p.x + (lane < 8 ? c_ring1x[lane] : c_ring2x[lane - 8]);
+        const int ny = p.y + (lane < 8 ? c_ring1y[lane] : c_ring2y[lane - 8]);
+        int ok = nx >= 0 && nx < W && ny >= 0 && ny < H;
+        if (ok && gate) {
+            float2 sf = make_float2(0.0f, 0.0f), nf = make_float2(0.0f, 0.0f);
+            if (p.z != D - 1) {
+                sf = flow_gl(fwd, W, H, p.x, p.y, p.z);
+                nf = flow_gl(fwd, W, H, nx, ny, p.z);
+            }
+            ok = angle_gate(sf.x, sf.y, nf.x, nf.y);
+        }
+        if (ok) {
+            float m[6];
+            ok = voxel_metric(fwd, bwd, W, H, D, nx, ny, p.z, m);
+            for (int k = 0; k < 6; ++k) s_m[lane][k] = m[k];
+        }
+        s_ok[lane] = ok;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    float sum[6] = {0, 0, 0, 0, 0, 0};
+    int cnt = 0;
+    for (int ring = 0; ring < 2 && cnt == 0; ++ring) {
+        const int b = ring == 0 ? 0 : 8, e = ring == 0 ? 8 : 24;
+        for (int j = b; j < e; ++j) {
+            if (!s_ok[j]) continue;
+            for (int k = 0; k < 6; ++k) sum[k] = sum[k] + s_m[j][k];
+            ++cnt;
+        }
+    }
+    float m[6];
+    if (cnt > 0) {
+        for (int k = 0; k < 6; ++k) m[k] = sum[k] / (float)cnt;
+    } else {
+        const float4 o0 = own[2 * i], o1 = own[2 * i + 1];
+        const float self[6] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y};
+        nearest_psd(self, m);
+    }
+    write_seed(i, p, slab[i], m, rec, trace_iter);
 }
 
 // ---------------------------------------------------------------------------
@@ -1404,115 +1470,117 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 // centroid reduction (centroids.vs:19-53)
 // ---------------------------------------------------------------------------
 
-// Bricks of 64 columns x ROWS rows x zc frames: each thread walks SEG
-// consecutive voxels of one row per frame, keeping a run of equal labels in
-// registers; for a run of n voxels from x = a to x = e on row (y, z) the
-// position sums are closed forms
-//   sum(2x+1) = (e+1)^2 - a^2,  sum(2y+1) = n(2y+1),  sum(2z+1) = n(2z+1),
-// and only L, a, b are accumulated.  A finished run is added into the
-// block's LDS table (open addressing by label, at most kRedProbes probes,
-// 64-bit LDS atomics), or straight into the global slot when its probes are
-// taken; the table goes to the global slots with one 64-bit atomic per sum
-// and label at the end.  Compact bricks keep the labels a block meets few,
-// and later frames of a brick mostly hit the slots of earlier ones.
+// Block-local slot table of the reduction: open addressing by label, at most
+// kRedProbes probes, else the run goes straight to its global slot.
 // Integer sums are exact, so no order of these additions changes a result.
 constexpr int kRedSlots = 256;
 constexpr int kRedProbes = 16;
-constexpr int kRedTileW = 64;
 
-__device__ __forceinline__ void run_flush(int label, int A, int xa, int xe, int y, int z, unsigned sl,
-                                          unsigned sa, unsigned sb, int* s_key,
-                                          unsigned long long (*s_acc)[8], unsigned long long* __restrict__ sums)
+// Inclusive prefix sum over the 64 lanes of a wave (DPP: row shifts, then
+// the row broadcasts of lanes 15 and 31).
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x)
 {
-    const int key = label < 0 ? A : label;
-    const unsigned long long n = (unsigned long long)(xe - xa + 1);
-    unsigned long long vals[7];
-    vals[0] = (unsigned long long)(xe + 1) * (unsigned long long)(xe + 1) - (unsigned long long)xa * (unsigned long long)xa;
-    vals[1] = n * (unsigned long long)(2 * y + 1);
-    vals[2] = n * (unsigned long long)(2 * z + 1);
-    vals[3] = n;
-    vals[4] = sl;
-    vals[5] = sa;
-    vals[6] = sb;
-    unsigned h = (hash_slot(key) << 1) & (kRedSlots - 1);
-    for (int p = 0; p < kRedProbes; ++p) {
-        const int old = atomicCAS(&s_key[h], kEmpty, key);
-        if (old == kEmpty || old == key) {
-#pragma unroll
-            for (int i = 0; i < 7; ++i) atomicAdd(&s_acc[h][i], vals[i]);
-            return;
-        }
-        h = (h + 1) & (kRedSlots - 1);
-    }
-    unsigned long long* o = sums + (size_t)key * kSumWords;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) atomicAdd(o + i, vals[i]);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return x;
 }
 
-template <int SEG>
-__global__ __launch_bounds__(256) void k_reduce_bricks(int W, int H, int D, int A, int tiles_x, int tiles_y, int zc,
-                                                       const int32_t* __restrict__ labels,
-                                                       const uint32_t* __restrict__ lab,
-                                                       unsigned long long* __restrict__ sums)
+// Rows as wave-wide runs: a wave reads 64 consecutive voxels of a row (one
+// per lane, coalesced), finds the runs of equal labels with a ballot of run
+// heads, and takes each run's LAB sums as differences of two wave prefix
+// sums (L and a packed in one word: each field's sum over 64 voxels stays
+// below 2^16).  Only a run's last lane adds it into the block's LDS table,
+// as 32-bit sums (a brick of 64 x kRowBrick x zc voxels, zc <= 8, keeps
+// every slot's sums below 2^32 for W, H, D < 65536); the table goes to the
+// global u64 slots at the end.  Position sums are closed forms: a run of n
+// voxels from x = a to x = e on row (y, z) adds
+//   sum(2x+1) = (e+1)^2 - a^2,  sum(2y+1) = n(2y+1),  sum(2z+1) = n(2z+1).
+constexpr int kRowBrick = 16;
+
+__global__ __launch_bounds__(256) void k_reduce_rows(int W, int H, int D, int A, int tiles_x, int tiles_y, int zc,
+                                                     const int32_t* __restrict__ labels,
+                                                     const uint32_t* __restrict__ lab,
+                                                     unsigned long long* __restrict__ sums)
 {
-    constexpr int TPR = kRedTileW / SEG;           // threads per row
-    constexpr int ROWS = 256 / TPR;
     __shared__ int s_key[kRedSlots];
-    __shared__ unsigned long long s_acc[kRedSlots][8];
+    __shared__ uint32_t s_acc[kRedSlots][8];
     const int tid = threadIdx.x;
     s_key[tid] = kEmpty;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) s_acc[tid][i] = 0ull;
+    for (int i = 0; i < 8; ++i) s_acc[tid][i] = 0u;
     __syncthreads();
 
     const int b = blockIdx.x;
     const int tx = b % tiles_x, ty = (b / tiles_x) % tiles_y, tz = b / (tiles_x * tiles_y);
-    const int xs = tx * kRedTileW + (tid % TPR) * SEG;
-    const int y = ty * ROWS + tid / TPR;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int x0 = tx * 64, x = x0 + lane;
+    const int nvalid = min(64, W - x0);
+    const bool valid = lane < nvalid;
     const int z0 = tz * zc, z1 = min(D, z0 + zc);
-    const int nval = min(SEG, W - xs);
-    if (y < H && nval > 0) {
-        const bool vec = nval == SEG && (W & 3) == 0;          // 16-byte aligned rows
-        for (int z = z0; z < z1; ++z) {
-            const size_t v0 = ((size_t)z * H + y) * W + xs;
-            int lv[SEG];
-            uint32_t cv[SEG];
-            if (vec) {
-                const int4* lp = reinterpret_cast<const int4*>(labels + v0);
-                const uint4* cp = reinterpret_cast<const uint4*>(lab + v0);
+    constexpr int RPW = kRowBrick / 4;           // rows per wave and frame
+    for (int z = z0; z < z1; ++z) {
+        int lv[RPW];
+        uint32_t cv[RPW];
 #pragma unroll
-                for (int q = 0; q < SEG / 4; ++q) {
-                    const int4 a = lp[q];
-                    const uint4 c = cp[q];
-                    lv[4 * q] = a.x; lv[4 * q + 1] = a.y; lv[4 * q + 2] = a.z; lv[4 * q + 3] = a.w;
-                    cv[4 * q] = c.x; cv[4 * q + 1] = c.y; cv[4 * q + 2] = c.z; cv[4 * q + 3] = c.w;
-                }
-            } else {
+        for (int j = 0; j < RPW; ++j) {
+            const int y = ty * kRowBrick + wave + 4 * j;
+            const size_t v = ((size_t)z * H + y) * W + x;
+            const bool ok = valid && y < H;
+            lv[j] = ok ? labels[v] : 0;
+            cv[j] = ok ? lab[v] : 0u;
+        }
 #pragma unroll
-                for (int q = 0; q < SEG; ++q) {
-                    const bool ok = q < nval;
-                    lv[q] = ok ? labels[v0 + q] : 0;
-                    cv[q] = ok ? lab[v0 + q] : 0u;
-                }
-            }
-            int cur = lv[0];
-            int xa = xs;
-            unsigned sl = cv[0] & 255u, sa = (cv[0] >> 8) & 255u, sb = (cv[0] >> 16) & 255u;
+        for (int j = 0; j < RPW; ++j) {
+            const int y = ty * kRowBrick + wave + 4 * j;
+            if (y >= H) break;                   // uniform
+            const int l = lv[j];
+            const uint32_t c = cv[j];
+            const int left = __shfl_up(l, 1);
+            const bool head = valid && (lane == 0 || l != left);
+            const unsigned long long heads = __ballot(head);
+            const bool tail = valid && (lane == nvalid - 1 || ((heads >> (lane + 1)) & 1ull));
+            const uint32_t pla = wave_scan_incl((c & 255u) | (((c >> 8) & 255u) << 16));
+            const uint32_t pb = wave_scan_incl((c >> 16) & 255u);
+            const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
+            const int h = upto ? 63 - __clzll((long long)upto) : 0;
+            const int src = h > 0 ? h - 1 : 0;
+            const uint32_t qla = __shfl(pla, src), qb = __shfl(pb, src);
+            if (tail) {
+                const uint32_t sla = h > 0 ? pla - qla : pla;
+                const uint32_t sb = h > 0 ? pb - qb : pb;
+                const uint32_t n = (uint32_t)(lane - h + 1);
+                const uint32_t xa = (uint32_t)(x0 + h), xe1 = (uint32_t)(x + 1);
+                uint32_t vals[7];
+                vals[0] = xe1 * xe1 - xa * xa;
+                vals[1] = n * (uint32_t)(2 * y + 1);
+                vals[2] = n * (uint32_t)(2 * z + 1);
+                vals[3] = n;
+                vals[4] = sla & 0xFFFFu;
+                vals[5] = sla >> 16;
+                vals[6] = sb;
+                const int key = l < 0 ? A : l;
+                unsigned hs = (hash_slot(key) << 1) & (kRedSlots - 1);
+                bool done = false;
+                for (int p = 0; p < kRedProbes; ++p) {
+                    const int old = atomicCAS(&s_key[hs], kEmpty, key);
+                    if (old == kEmpty || old == key) {
 #pragma unroll
-            for (int q = 1; q < SEG; ++q) {
-                if (q < nval) {
-                    const int l = lv[q];
-                    if (l != cur) {
-                        run_flush(cur, A, xa, xs + q - 1, y, z, sl, sa, sb, s_key, s_acc, sums);
-                        cur = l;
-                        xa = xs + q;
-                        sl = 0u; sa = 0u; sb = 0u;
+                        for (int i = 0; i < 7; ++i) atomicAdd(&s_acc[hs][i], vals[i]);
+                        done = true;
+                        break;
                     }
-                    const uint32_t c = cv[q];
-                    sl += c & 255u; sa += (c >> 8) & 255u; sb += (c >> 16) & 255u;
+                    hs = (hs + 1) & (kRedSlots - 1);
+                }
+                if (!done) {
+                    unsigned long long* o = sums + (size_t)key * kSumWords;
+#pragma unroll
+                    for (int i = 0; i < 7; ++i) atomicAdd(o + i, (unsigned long long)vals[i]);
                 }
             }
-            run_flush(cur, A, xa, xs + nval - 1, y, z, sl, sa, sb, s_key, s_acc, sums);
         }
     }
     __syncthreads();
@@ -1520,7 +1588,7 @@ __global__ __launch_bounds__(256) void k_reduce_bricks(int W, int H, int D, int 
     if (key != kEmpty) {
         unsigned long long* o = sums + (size_t)key * kSumWords;
 #pragma unroll
-        for (int i = 0; i < 7; ++i) atomicAdd(o + i, s_acc[tid][i]);
+        for (int i = 0; i < 7; ++i) atomicAdd(o + i, (unsigned long long)s_acc[tid][i]);
     }
 }
 
@@ -1578,21 +1646,21 @@ __global__ void k_finalize(int W, int H, int D, int A, const unsigned long long*
 
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
-hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared,
-                          const float2* fwd, const float2* bwd, const float* lin_lut,
-                          uint32_t* lab, float4* field, int* fix_list, int* fix_count)
+hipError_t launch_prepare(hipStream_t st, const Dims& d, const uint8_t* rgb, int prepared, const float* lin_lut,
+                          uint32_t* lab)
 {
     dim3 block(64, 4, 1);
     dim3 grid(cdiv(d.W, 64), cdiv(d.H, 4), (unsigned)d.D);
-    hipLaunchKernelGGL(k_prepare, grid, block, 0, st, d.W, d.H, d.D, rgb, prepared, fwd, bwd, lin_lut,
-                       lab, field, fix_list, fix_count);
+    hipLaunchKernelGGL(k_prepare, grid, block, 0, st, d.W, d.H, d.D, rgb, prepared, lin_lut, lab);
     return hipGetLastError();
 }
 
-hipError_t launch_fixup(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd,
-                        float4* field, const int* fix_list, const int* fix_count)
+hipError_t launch_metric_field(hipStream_t st, const Dims& d, const float2* fwd, const float2* bwd, float4* field)
 {
-    hipLaunchKernelGGL(k_fixup, dim3(1024), dim3(64), 0, st, d.W, d.H, d.D, fwd, bwd, field, fix_list, fix_count);
+    const int64_t V = (int64_t)d.W * d.H * d.D;
+    unsigned blocks = V > 0 ? cdiv(V, 256) : 1;
+    if (blocks > 65536u) blocks = 65536u;
+    hipLaunchKernelGGL(k_metric_field, dim3(blocks), dim3(256), 0, st, d.W, d.H, d.D, fwd, bwd, field);
     return hipGetLastError();
 }
 
@@ -1615,12 +1683,14 @@ hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, con
     return hipGetLastError();
 }
 
-hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab,
-                              const float4* field, const float2* fwd, int gate, SeedRec* rec,
-                              float* trace_iter)
+hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab, const float2* fwd,
+                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter)
 {
-    hipLaunchKernelGGL(k_seed_metric, dim3(cdiv(d.A, 64)), dim3(64), 0, st, d.W, d.H, d.D, d.A, pos, slab,
-                       field, fwd, gate, rec, trace_iter);
+    if (d.A <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seed_metric, dim3(cdiv(d.A, 64)), dim3(64), 0, st, d.W, d.H, d.D, d.A, pos, slab, fwd, bwd,
+                       rec, own, need, trace_iter);
+    hipLaunchKernelGGL(k_seed_metric_rings, dim3((unsigned)d.A), dim3(64), 0, st, d.W, d.H, d.D, pos, slab, fwd,
+                       bwd, gate, rec, own, need, trace_iter);
     return hipGetLastError();
 }
 
@@ -1931,28 +2001,18 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums)
 {
-    // the widest rows per thread (16, 8 or 4 voxels) that still give about
-    // four blocks per CU, then frames per brick to keep the block count
-    // near 2048 on long clips
-    const int tiles_x = (int)cdiv(d.W, kRedTileW);
-    int seg = 16;
-    while (seg > 4 && (int64_t)tiles_x * cdiv(d.H, 256 / (kRedTileW / seg)) * d.D < 1024) seg >>= 1;
-    const int rows = 256 / (kRedTileW / seg);
-    const int tiles_y = (int)cdiv(d.H, rows);
+    if (d.W > 65535 || d.H > 65535 || d.D > 65535) return hipErrorInvalidValue;   // 32-bit brick sums
+    // bricks of 64 columns x kRowBrick rows x zc frames, zc chosen to keep
+    // the block count near 2048 on long clips
+    const int tiles_x = (int)cdiv(d.W, 64);
+    const int tiles_y = (int)cdiv(d.H, kRowBrick);
     const int64_t per_frame = (int64_t)tiles_x * tiles_y;
     int zc = (int)(per_frame * d.D / 2048);
     zc = zc < 1 ? 1 : (zc > 8 ? 8 : zc);
     const int64_t nb = per_frame * cdiv(d.D, zc);
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (seg == 16)
-        hipLaunchKernelGGL(k_reduce_bricks<16>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
-                           tiles_y, zc, labels, lab, sums);
-    else if (seg == 8)
-        hipLaunchKernelGGL(k_reduce_bricks<8>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
-                           tiles_y, zc, labels, lab, sums);
-    else
-        hipLaunchKernelGGL(k_reduce_bricks<4>, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x,
-                           tiles_y, zc, labels, lab, sums);
+    hipLaunchKernelGGL(k_reduce_rows, dim3((unsigned)nb), dim3(256), 0, st, d.W, d.H, d.D, d.A, tiles_x, tiles_y, zc,
+                       labels, lab, sums);
     return hipGetLastError();
 }
 

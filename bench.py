@@ -1,8 +1,9 @@
 """Benchmark: plain JFA + Lloyd supervoxel run on one MI355X.
 
 One step = one whole plain run of the hot path over the resident clip:
-LAB + metric construction, then `iters` Lloyd iterations (seed stage, P JFA
-passes, reduction, finalisation), ending with the label volume in HBM.
+LAB construction, then `iters` Lloyd iterations (seed stage with the
+per-voxel metric evaluated at the seeds and their rings, P JFA passes,
+reduction, finalisation), ending with the label volume in HBM.
 Default workload: 640x480x200 synthetic video (seed 2, 32 ellipses,
 |v| <= 4 px/frame, noise sigma 6), K = 10000 (9720 placed), 10 iterations,
 reference JFA schedule 1,9,5,3,2,1,2,1 (P = 8), FP32 metric, int32 labels.
@@ -118,7 +119,7 @@ def batched(n_clips, steps):
     """Batched workload: n_clips clips from the generator (video seeds
     100.., sizes {160x120, 320x240, 480x360, 640x480}, 20-120 frames,
     K 200-4000, 8-32 ellipses, 10 Lloyd iterations; see batch.py), resident
-    on the device.  Times `steps` whole runs (LAB + metric and 10 iterations
+    on the device.  Times `steps` whole runs (LAB and 10 iterations
     of every clip) of the plain per-clip loop (asv_run clip after clip on the
     same contexts) and of the batched form (asv_batch_run: graphs, batched
     marching launches, autotuned shapes; tuning happens in an untimed first
@@ -245,7 +246,7 @@ def main():
             "workload": args.workload,
             "W": W, "H": H, "D": D, "voxels": V, "K": K, "seeds_placed": clip.A,
             "lloyd_iters": iters, "jfa_schedule": steps_list, "jfa_ratio_mult": mult,
-            "tie_rule": "lowest-id", "step_includes": "LAB+metric, seed stages, JFA, reduction, finalize",
+            "tie_rule": "lowest-id", "step_includes": "LAB, seed stages (metric at the seeds), JFA, reduction, finalize",
         },
         "roofline": {
             "bound": "hbm",

@@ -120,14 +120,18 @@ int asv_upload_frames(asv_ctx* ctx, const uint8_t* rgb, void* stream);
  * Frame D-1 of fwd and frame 0 of bwd are ignored, as in the reference. */
 int asv_upload_flow(asv_ctx* ctx, const float* fwd, const float* bwd, void* stream);
 
-/* Image::RGB2LAB (GPU-AS/Image.cpp:296-315) fused with the per-voxel
- * matrix_slover metric (GPU-AS/Image.cpp:874-1009): builds the LAB volume and
- * the metric field. */
+/* Image::RGB2LAB (GPU-AS/Image.cpp:296-315) with the frame preprocessing of
+ * Render::cvMat2TexureInput (GPU-AS/Render.cpp:354-359): builds the LAB
+ * volume.  The per-voxel matrix_slover metric (GPU-AS/Image.cpp:874-1009) is
+ * evaluated by asv_seed_stage at the seeds and their rings, as the
+ * reference does, so no per-voxel field is kept. */
 int asv_prepare(asv_ctx* ctx, void* stream);
 
 /* Image::Ld_first_init (GPU-AS/Image.cpp:1017-1045) for iteration 1 and
  * Image::Ld_next_seeds_matrix (GPU-AS/Image.cpp:1300-1575) afterwards: seed
- * metric selection and scatter into the label volume. */
+ * metric selection (seed_matrix / matrix_slover at the seed voxel, then the
+ * ring fallbacks, GPU-AS/Image.cpp:689-1009) and scatter into the label
+ * volume. */
 int asv_seed_stage(asv_ctx* ctx, int32_t iter, void* stream);
 
 /* Image::flood (GPU-AS/Image.cpp:1061-1190): the P JFA passes. */
@@ -215,7 +219,8 @@ int asv_enable_trace(asv_ctx* ctx, int32_t on);
 int asv_download_trace(asv_ctx* ctx, float* trace, void* stream);
 
 /* LAB volume (D x H x W packed L | a<<8 | b<<16) and metric field
- * (D x H x W x 8 float: m0..m5, psd flag, 0), for tests and tools. */
+ * (D x H x W x 8 float: m0..m5, psd flag, 0, computed on request into a
+ * temporary device buffer), for tests and tools. */
 int asv_download_lab(asv_ctx* ctx, uint32_t* lab, void* stream);
 int asv_download_metric(asv_ctx* ctx, float* m8, void* stream);
 
