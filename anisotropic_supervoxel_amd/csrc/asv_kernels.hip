@@ -29,6 +29,11 @@ namespace asv {
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
 
+// A byte of LDS read on its own: taps of the JFA passes that are adjacent
+// bytes (step 1) would otherwise be merged by the compiler into 16-bit reads
+// at odd LDS addresses, which are misaligned and slow.
+using lds_byte = const volatile __attribute__((address_space(3))) uint8_t;
+
 // One 16-byte record field through a global pointer.
 __device__ __forceinline__ float4 ld_f4(gptr<const float> p)
 {
@@ -1067,13 +1072,14 @@ __global__ __launch_bounds__(256) void k_jfa_tile(int W, int H, int D, int tiles
 
     if (!overflow) {
         unsigned long long mask = 0ull;
+        lds_byte* pidx = (lds_byte*)(s_idx + base);
 #pragma unroll
         for (int sl = 0; sl < 3; ++sl)
 #pragma unroll
             for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
                 for (int dx = -1; dx <= 1; ++dx)
-                    mask |= 1ull << s_idx[sl * RA + base + dy * K * R + dx * K];
+                    mask |= 1ull << pidx[sl * RA + dy * K * R + dx * K];
         mask &= ~(1ull << kMaxCand);
         while (mask) {
             int j = __ffsll((long long)mask) - 1;
@@ -1310,9 +1316,12 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
         };
+        // the taps are read byte by byte (lds_byte): at K = 1 the 16-bit
+        // reads the compiler would merge them into took the pass from 335
+        // to 595 us at 640x480x200
         auto slab_mask = [&](int slot) {
             MaskT m = 0;
-            const uint8_t* p = s_idx + slot * RA + mbase;
+            lds_byte* p = (lds_byte*)(s_idx + slot * RA + mbase);
 #pragma unroll
             for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
@@ -1756,9 +1765,10 @@ int march_s(int k)
         forced = (v == 1 || v == 2) ? v : 0;
     }
     if (forced) return forced;
-    // measured at 640x480x200: K = 1 593 vs 608 us, K = 2 420 vs 383,
+    // measured at 640x480x200: K = 1 362 vs 335 us, K = 2 420 vs 383,
     // K = 3 622 vs 566, K = 5 573 vs 511 (one vs two slices per step)
-    return k >= 2 ? 2 : 1;
+    (void)k;
+    return 2;
 }
 
 using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
