@@ -1122,6 +1122,10 @@ template <int MC> struct MarchMask;
 template <> struct MarchMask<31> { using T = uint32_t; };
 template <> struct MarchMask<63> { using T = unsigned long long; };
 
+// A staged id after the claim: the compact index it resolved to, tagged
+// with this bit (ids of the marching pass stay below kMarchMaxIds < 2^30).
+constexpr int kResolved = 0x40000000;
+
 // Slices of residue r (z = r, r + K, ...) in segment g of a clip.
 template <int K>
 __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
@@ -1171,12 +1175,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     using MaskT = typename MarchMask<MC>::T;
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
     constexpr int RA = R * R;
-    constexpr int RPI = 64 / R;               // staged rows per wave instruction
-    constexpr int NRS = (R + 4 * RPI - 1) / (4 * RPI);   // instructions per wave per slab
+    constexpr int NRS = (RA + 255) / 256;     // staged elements per thread and slab
     constexpr MaskT kBadBit = (MaskT)1 << MC;
     // LAB of the next slice loaded during this one; at K = 2 the extra
     // register costs a block per CU, more than the latency it hides
     constexpr bool kLabAhead = K != 2;
+    // Claims that also resolve the ids already listed (see claim below):
+    // fewer table reads, four to eight more registers, which cost a wave per
+    // SIMD above K = 3 (measured at 640x480x200, K = 1 327 vs 340 us, K = 3
+    // 529 vs 548, K = 5 510 vs 444)
+    constexpr bool kResolveInClaim = K <= 3;
     extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
     __shared__ uint8_t s_idx[(S + 2) * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
@@ -1198,8 +1206,6 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     const int i_end = i_beg + qx + (xcd < rx ? 1 : 0);
 
     const int wave = tid >> 6, lane = tid & 63;
-    const int sub = lane / R, col = lane - (lane / R) * R;
-    const bool lane_ok = sub < RPI;
     const int lx = (wave & 1) * 8 + (lane & 7);
     const int ly = (wave >> 1) * 8 + (lane >> 3);
     const int mbase = (ly + K) * R + (lx + K);
@@ -1240,7 +1246,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const float ratio = c.ratio;
         const int tx = t % c.tiles_x, ty = t / c.tiles_x;
         const int x0 = tx * kTile, y0 = ty * kTile;
-        const int gx = clampi(x0 - K + col, 0, W - 1);
+        // a slab is staged in raster order over the grown tile: thread tid
+        // holds elements tid, tid + 256, ... (offsets within a frame, -1 for
+        // none), so every lane of every wave stages
+        int soff[NRS];
+#pragma unroll
+        for (int i = 0; i < NRS; ++i) {
+            const int f = tid + 256 * i;
+            const int rr = f / R, cc = f - (f / R) * R;
+            soff[i] = f < RA ? clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1) : -1;
+        }
         const int x = x0 + lx, y = y0 + ly;
         const bool mine = x < W && y < H;
         const float fx = (float)x, fy = (float)y;
@@ -1248,46 +1263,54 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         auto load_slab = [&](int gz, int* ids) {
             const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
 #pragma unroll
-            for (int i = 0; i < NRS; ++i) {
-                const int rr = (wave + 4 * i) * RPI + sub;
-                int id = -1;
-                if (lane_ok && rr < R) id = slab[clampi(y0 - K + rr, 0, H - 1) * W + gx];
-                ids[i] = id;
-            }
+            for (int i = 0; i < NRS; ++i) ids[i] = soff[i] >= 0 ? slab[soff[i]] : -1;
         };
-        auto claim = [&](const int* ids) {
+        // The first lane of each run of equal ids claims an unclaimed id
+        // (list position, published index byte).  With kResolveInClaim
+        // every staged id reads its table byte here, and an id already
+        // listed resolves to its compact index at once (kept in ids[i] with
+        // the kResolved tag); the rest are looked up after the barrier.
+        auto claim = [&](int* ids) {
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
                 const int id = ids[i];
                 const int left = __builtin_amdgcn_update_dpp(-2, id, 0x111, 0xF, 0xF, false);
-                if (id >= 0 && id != left) {
+                if (id >= 0 && (kResolveInClaim || id != left)) {
                     const unsigned sh = (unsigned)(id & 3) * 8u;
                     uint32_t old = s_slot[id >> 2];
-                    while (((old >> sh) & 0xFFu) == kSlotEmpty) {
-                        const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
-                        const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
-                        if (prev == old) {
-                            const int k = atomicAdd(&s_count, 1);
-                            if (k < kListCap) s_list[k] = id;
-                            const unsigned byte = k < MC ? (unsigned)k : (unsigned)MC;
-                            atomicAnd(&s_slot[id >> 2], ~(0xFFu << sh));
-                            atomicOr(&s_slot[id >> 2], byte << sh);
-                            break;
+                    unsigned byte = (old >> sh) & 0xFFu;
+                    if (byte == kSlotEmpty && id != left) {
+                        while (((old >> sh) & 0xFFu) == kSlotEmpty) {
+                            const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
+                            const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
+                            if (prev == old) {
+                                const int k = atomicAdd(&s_count, 1);
+                                if (k < kListCap) s_list[k] = id;
+                                byte = k < MC ? (unsigned)k : (unsigned)MC;
+                                // one atomic turns the claimed byte into the
+                                // index: lanes reading the word meanwhile see
+                                // either the claim or the index
+                                atomicXor(&s_slot[id >> 2], (kSlotClaimed ^ byte) << sh);
+                                break;
+                            }
+                            old = prev;
+                            byte = (old >> sh) & 0xFFu;
                         }
-                        old = prev;
                     }
+                    if (kResolveInClaim && byte < kSlotClaimed) ids[i] = kResolved | (int)byte;
                 }
             }
         };
         auto convert = [&](const int* ids, int slot) {
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
-                const int rr = (wave + 4 * i) * RPI + sub;
-                const int id = ids[i];
-                if (lane_ok && rr < R) {
+                const int f = tid + 256 * i;
+                const int v = ids[i];
+                if (f < RA) {
                     unsigned cix = (unsigned)MC;
-                    if (id >= 0) cix = (s_slot[id >> 2] >> ((unsigned)(id & 3) * 8u)) & 0xFFu;
-                    s_idx[slot * RA + rr * R + col] = (uint8_t)cix;
+                    if (v >= kResolved) cix = (unsigned)v & 0xFFu;
+                    else if (v >= 0) cix = (s_slot[v >> 2] >> ((unsigned)(v & 3) * 8u)) & 0xFFu;
+                    s_idx[slot * RA + f] = (uint8_t)cix;
                 }
             }
         };
@@ -1316,7 +1339,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
         };
-        // the taps are read byte by byte (lds_byte): at K = 1 the 16-bit
+        // index bytes are read one by one (lds_byte): at K = 1 the 16-bit
         // reads the compiler would merge them into took the pass from 335
         // to 595 us at 640x480x200
         auto slab_mask = [&](int slot) {
@@ -1749,7 +1772,9 @@ int march_mc(int k)
         forced = (v == 31 || v == 63) ? v : 0;
     }
     if (forced) return forced;
-    return k <= 3 ? 31 : 63;
+    // measured at 640x480x200: K = 3 536 (31) vs 530 us (63), K = 5 437 vs
+    // 483 us
+    return k <= 5 ? 31 : 63;
 }
 
 // Slices of a chain evaluated per step of the marching pass at step k: two
