@@ -860,13 +860,18 @@ __global__ __launch_bounds__(256) void k_jfa_seeds2(int W, int H, int D, int k1,
 // candidate to v's 64-bit key (distance bits, id) by atomic minimum --
 // the minimum over v's labelled taps is the full pass's lowest-id winner,
 // and repeats (a tap reached twice, or from a neighbouring seed's block)
-// cannot change a minimum; mode 1: v takes the label of its key (-1 while
-// the key is still the initial one, d >= 1e8); mode 2: the keys touched
-// go back to the initial value for the next use (a separate launch: a
-// repeat of v must still read its key in mode 1; folding the reset into
-// mode 1 as an atomic exchange measured 2x slower than the two plain
-// launches).  A tap cell is read once per pair, instead of 27 scattered
-// taps per voxel.
+// cannot change a minimum; mode 1: the block of the winning seed w (the id
+// in v's key) writes w into v and puts the key back to its initial value
+// for the next use.  Every other block that meets v skips it, whether it
+// reads the key before or after the reset, so each key is resolved and
+// reset exactly once with plain loads and stores.  The winner's block does
+// meet v: w's label sits at a tap t of v that lies in w's step-k1 set,
+// since the first pass gave t the label of a seed among its own taps.  A
+// voxel without a candidate keeps the seed stage's -1 (a seed voxel always
+// has one: its own first-pass label).  A v that occurs in several pairs
+// of its owner's block (small steps, clamped edges) is written with the
+// same value by whichever pair sees the key first.  A tap cell is read once
+// per pair, instead of 27 scattered taps per voxel.
 constexpr int kSeed2Pairs = (kSeedPassMaxStep + 1) * (kSeedPassMaxStep + 1);
 
 __global__ __launch_bounds__(256) void k_jfa_seeds2_scatter(int W, int H, int D, int k1, int k2, int mode,
@@ -912,11 +917,12 @@ __global__ __launch_bounds__(256) void k_jfa_seeds2_scatter(int W, int H, int D,
                                       ratio);
             const unsigned long long kk = dkey(d, id);
             if (kk < ASV_KEY_INIT) atomicMin(keys + v, kk);
-        } else if (mode == 1) {
-            const unsigned long long kk = keys[v];
-            out[v] = kk >= ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(kk & 0xffffffffull);
         } else {
-            keys[v] = ASV_KEY_INIT;
+            const unsigned long long kk = keys[v];
+            if (kk < ASV_KEY_INIT && (unsigned)(kk & 0xffffffffull) == (unsigned)blockIdx.x) {
+                out[v] = (int32_t)blockIdx.x;
+                keys[v] = ASV_KEY_INIT;
+            }
         }
     }
 }
@@ -1189,7 +1195,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     __shared__ uint8_t s_idx[(S + 2) * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
     __shared__ float4 s_u[S][MC];             // (dz 2m2, dz 2m4, dz^2 m5, m3) at the step's slices
-    __shared__ float4 s_c[MC];                // (L, a, b, id bits)
+    __shared__ float4 s_c[MC];                // (id bits, L, a, b)
     __shared__ float4 s_z[MC];                // (z, 2 m2, 2 m4, m5)
     __shared__ int s_list[kListCap];
     __shared__ int s_count;
@@ -1325,7 +1331,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     const float4 a = ld_f4(q), b = ld_f4(q + 4), cc = ld_f4(q + 8);
                     s_p[e] = make_float4(a.x, a.y, a.w, b.x);
                     s_z[e] = make_float4(a.z, b.y, b.w, cc.x);
-                    s_c[e] = make_float4(cc.y, cc.z, cc.w, __int_as_float(id));
+                    s_c[e] = make_float4(__int_as_float(id), cc.y, cc.z, cc.w);
                     sz = a.z; m2d = b.y; m4d = b.w; m5 = cc.x; m3 = b.z;
                 } else {
                     const float4 zz = s_z[e];
@@ -1458,10 +1464,10 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
                             const float t2 = __fmaf_rn(dy, u.w, u.y);
                             const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
-                            const float c0 = l0 - cc.x, c1 = l1 - cc.y, c2 = l2 - cc.z;
+                            const float c0 = l0 - cc.y, c1 = l1 - cc.z, c2 = l2 - cc.w;
                             const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
                             const unsigned long long kk =
-                                ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.w);
+                                ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.x);
                             best = kk < best ? kk : best;
                         }
                     } else {
@@ -1979,7 +1985,7 @@ hipError_t launch_jfa_seeds2(hipStream_t st, const Dims& d, const int32_t* in, i
     // `out` is the seed stage's volume: -1 except at the seed voxels, and each
     // seed voxel lies in its own block's product set, so no fill is needed
     if (tie_rule == 0 && keys) {
-        for (int mode = 0; mode < 3; ++mode)
+        for (int mode = 0; mode < 2; ++mode)
             hipLaunchKernelGGL(k_jfa_seeds2_scatter, dim3((unsigned)d.A), dim3(256), 0, st, d.W, d.H, d.D, k1, k2, mode,
                                pos, in, out, lab, rec, ratio, keys);
     } else if (tie_rule == 0)
