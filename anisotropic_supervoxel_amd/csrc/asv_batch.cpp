@@ -130,7 +130,10 @@ int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
         const int32_t* in = c->labels[g.p & 1];
         int32_t* out = c->labels[1 - (g.p & 1)];
         switch (c->pass_kind((size_t)g.p)) {       // every iteration starts with the seed stage
-        case 1: BHIP(launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, g.K, ratio, c->p.tie_rule)); break;
+        case 1:
+            BHIP(launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, g.K, ratio, c->p.tie_rule,
+                                  c->first_pass_fill()));
+            break;
         case 2:
             BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
                                    c->keys));

@@ -613,7 +613,9 @@ hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
     const int32_t* in = c->labels[c->cur];
     int32_t* out = c->labels[1 - c->cur];
     switch (pass_kind_now(c, i)) {
-    case 1: return launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], ratio, c->p.tie_rule);
+    case 1:
+        return launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], ratio, c->p.tie_rule,
+                                c->first_pass_fill());
     case 2:
         return launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], c->steps[1], ratio,
                                  c->p.tie_rule, c->keys);

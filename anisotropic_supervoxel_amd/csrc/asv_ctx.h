@@ -55,4 +55,12 @@ struct asv_ctx {
     // first's output when it is sparse (about 729 voxels per seed against
     // a third of the volume), 0 the full marching/tiled pass.
     int pass_kind(size_t i) const { return asv::seed_pass_kind(steps, d.A, V(), i); }
+
+    // Whether the first pass must fill its output with -1: not when the
+    // second pass is the scatter form, which reads the first pass's output
+    // only at the voxels the first pass writes (the seeds' step-1 sets).
+    bool first_pass_fill() const
+    {
+        return !(pass_kind(1) == 2 && p.tie_rule == ASV_TIE_LOWEST_ID && keys != nullptr);
+    }
 };

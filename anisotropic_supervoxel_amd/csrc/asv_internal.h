@@ -98,7 +98,7 @@ inline int seed_pass_kind(const std::vector<int>& steps, int64_t A, int64_t V, s
     return 0;
 }
 hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule);
+                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule, bool fill = true);
 // The marching pass over several clips at once (step k <= 16, lowest-id
 // tie, A <= kMarchMaxIds per clip).  march_plan fills seg, nseg and
 // item_begin of the n host descriptors (per_block > 0: about per_block

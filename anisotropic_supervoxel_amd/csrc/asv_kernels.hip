@@ -360,23 +360,61 @@ __device__ __forceinline__ uint3 frame_rgb(const uint8_t* __restrict__ rgb, int 
     return make_uint3((ar + 8u) >> 4, (ag + 8u) >> 4, (ab + 8u) >> 4);
 }
 
-// Kernel (a): blur + flip + LAB for one voxel per thread.
+// Kernel (a): blur + flip + LAB for one voxel per thread, 64 x 4 voxels of
+// one frame per block.  Raw frames: the six image rows the block's blur
+// reads (its four rows and one above and below, reflect-101 applied) are
+// staged in LDS over 66 columns as packed R | G << 8 | B << 16, once per
+// block; each thread then sums its nine taps with R and B side by side in
+// one word (a weighted sum of 9 bytes stays below 2^16).  The integer
+// result equals frame_rgb's.
 __global__ __launch_bounds__(256) void k_prepare(int W, int H, int D, const uint8_t* __restrict__ rgb,
                                                  int prepared, const float* __restrict__ lin_lut,
                                                  uint32_t* __restrict__ lab)
 {
     __shared__ float lin[256];
-    int tid = threadIdx.y * 64 + threadIdx.x;
+    __shared__ uint32_t s_px[6][66];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = ty * 64 + tx;
     lin[tid] = lin_lut[tid];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, t = blockIdx.z;
+    if (!prepared) {
+        // staged row j is image row H - y0 - j; GL row y0 + ty is image row
+        // H - y0 - (ty + 1), so its taps are staged rows ty .. ty + 2.  The
+        // clamps only matter for rows and columns no valid voxel reads.
+        const uint8_t* fr = rgb + (size_t)t * H * W * 3;
+        for (int e = tid; e < 6 * 66; e += 256) {
+            const int j = e / 66, c = e - (e / 66) * 66;
+            const int yy = clampi(reflect101(H - y0 - j, H), 0, H - 1);
+            const int xx = clampi(reflect101(x0 - 1 + c, W), 0, W - 1);
+            const uint8_t* q = fr + ((size_t)yy * W + xx) * 3;
+            s_px[j][c] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16);
+        }
+    }
     __syncthreads();
 
-    int x = blockIdx.x * 64 + threadIdx.x;
-    int y = blockIdx.y * 4 + threadIdx.y;
-    int t = blockIdx.z;
+    const int x = x0 + tx, y = y0 + ty;
     if (x >= W || y >= H) return;
     size_t v = ((size_t)t * H + y) * W + x;
-    const uint3 c3 = frame_rgb(rgb, prepared, W, H, x, y, t);
-    lab[v] = lab_pack(c3.x, c3.y, c3.z, lin);
+    uint32_t r, g, b;
+    if (prepared) {
+        const uint8_t* q = rgb + v * 3;
+        r = q[0]; g = q[1]; b = q[2];
+    } else {
+        uint32_t rb = 0u, gg = 0u;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t w = (j == 1 ? 2u : 1u) * (i == 1 ? 2u : 1u);
+                const uint32_t px = s_px[ty + j][tx + i];
+                rb += w * (px & 0x00FF00FFu);
+                gg += w * ((px >> 8) & 0xFFu);
+            }
+        r = ((rb & 0xFFFFu) + 8u) >> 4;
+        g = (gg + 8u) >> 4;
+        b = ((rb >> 16) + 8u) >> 4;
+    }
+    lab[v] = lab_pack(r, g, b, lin);
 }
 
 // The metric of every voxel (m0..m5, psd flag, 0 per voxel), for tests and
@@ -1959,9 +1997,11 @@ static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, 
 }
 
 hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule)
+                            const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule, bool fill)
 {
-    hipError_t e = launch_fill(st, out, (int64_t)d.W * d.H * d.D, -1);
+    // without the fill, `out` keeps stale labels outside the seeds' sets:
+    // only for a next pass that reads nothing else (the scatter form)
+    hipError_t e = fill ? launch_fill(st, out, (int64_t)d.W * d.H * d.D, -1) : hipSuccess;
     if (e != hipSuccess || d.A < 1) return e;
     const int m = k + 1 > 3 ? k + 1 : 3;
     const int64_t threads = (int64_t)d.A * m * m * m;
