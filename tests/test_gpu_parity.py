@@ -61,9 +61,8 @@ def test_lab_exhaustive_all_rgb(have_gpu):
 @pytest.mark.parametrize("seed,scale", [(0, 3.0), (1, 3.0), (2, 3.0), (3, 150.0)])
 def test_metric_field_random_flows(have_gpu, seed, scale):
     """Per-voxel metric (LSQ + PSD flag) on flows with discontinuities,
-    including rank-deficient corners, bit-exact.  Flows up to 150 px take
-    the double-precision accumulation of the normal equations (deltas
-    beyond 73), small ones the 32-bit integer one."""
+    including rank-deficient corners and flows up to 150 px, bit-exact (the
+    field is computed on request with the seed kernels' device function)."""
     rng = np.random.default_rng(seed)
     D, H, W = 6, 29, 37
     fwd = rng.uniform(-scale, scale, (D, H, W, 2)).astype(np.float32)
@@ -81,6 +80,50 @@ def test_metric_field_random_flows(have_gpu, seed, scale):
     assert np.array_equal(m8[..., :6].view(np.uint32), m6.view(np.uint32))
     assert np.array_equal((m8[..., 6] != 0).astype(np.uint8), psd)
     assert (psd == 0).any(), "case must exercise the non-PSD fallbacks"
+
+
+_RING1 = [(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)]
+_RING2 = [(x, -2) for x in range(-2, 3)] + [(-2, -1), (2, -1), (-2, 0), (2, 0), (-2, 1), (2, 1)] + \
+    [(x, 2) for x in range(-2, 3)]
+
+
+def _seed_metric_paths(psd, seeds):
+    """How many seeds take each branch of seed_matrix (GPU-AS/Image.cpp:
+    689-872) without the angle gate: own metric PSD, mean of the PSD 1-ring,
+    of the PSD 2-ring, nearest PSD of the own metric."""
+    D, H, W = psd.shape
+    count = [0, 0, 0, 0]
+    for x, y, z in seeds:
+        if psd[z, y, x]:
+            count[0] += 1
+            continue
+        for i, ring in enumerate((_RING1, _RING2)):
+            if any(psd[z, y + dy, x + dx] for dx, dy in ring if 0 <= x + dx < W and 0 <= y + dy < H):
+                count[1 + i] += 1
+                break
+        else:
+            count[3] += 1
+    return count
+
+
+@pytest.mark.parametrize("gate", [0, 1])
+def test_seed_metric_fallbacks_bit_exact(have_gpu, gate):
+    """Random flows (8 px) make most metrics non-PSD, so the seeds of the
+    first iteration take every branch of the seed metric -- own, 1-ring
+    mean, 2-ring mean, nearest PSD -- and later iterations run them through
+    the flow-angle gate when it is on.  Every iteration's seed records,
+    labels and sums bit-exact."""
+    rng = np.random.default_rng(23)
+    D, H, W, K, iters = 8, 40, 48, 400, 3
+    fwd = rng.uniform(-8.0, 8.0, (D, H, W, 2)).astype(np.float32)
+    bwd = rng.uniform(-8.0, 8.0, (D, H, W, 2)).astype(np.float32)
+    rgb = rng.integers(0, 256, (D, H, W, 3), dtype=np.uint8)
+    _, psd = orc.metric_field(fwd, bwd)
+    seeds, _ = orc.grid_seeds(W, H, D, K)
+    paths = _seed_metric_paths(psd, seeds)
+    assert all(n > 0 for n in paths), f"case must take every branch, got {paths}"
+    got = _clip_run(rgb, fwd, bwd, K, iters, angle_gate=gate)
+    _assert_run_equal(got, rgb, fwd, bwd, K, iters, angle_gate=gate)
 
 
 @pytest.mark.parametrize("seed,shape,K,iters,kw", [
