@@ -1229,6 +1229,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // SIMD above K = 3 (measured at 640x480x200, K = 1 327 vs 340 us, K = 3
     // 529 vs 548, K = 5 510 vs 444)
     constexpr bool kResolveInClaim = K <= 3;
+    // Straight-line prefetch (every step issues its slab and LAB loads
+    // unconditionally, clamped), so the compiler waits for just the loads a
+    // value needs instead of all of them: K = 2 373 -> 355 us, K = 3 508 ->
+    // 496; above K = 3 its registers cost a wave per SIMD (K = 5 436 -> 498)
+    constexpr bool kStraight = K <= 3;
     extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
     __shared__ uint8_t s_idx[(S + 2) * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
@@ -1306,8 +1311,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 
         auto load_slab = [&](int gz, int* ids) {
             const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
+            // kStraight: cells past the grown tile read cell 0 and drop it
 #pragma unroll
-            for (int i = 0; i < NRS; ++i) ids[i] = soff[i] >= 0 ? slab[soff[i]] : -1;
+            for (int i = 0; i < NRS; ++i) {
+                if (kStraight) {
+                    const int v = slab[soff[i] >= 0 ? soff[i] : 0];
+                    ids[i] = soff[i] >= 0 ? v : -1;
+                } else {
+                    ids[i] = soff[i] >= 0 ? slab[soff[i]] : -1;
+                }
+            }
         };
         // The first lane of each run of equal ids claims an unclaimed id
         // (list position, published index byte).  With kResolveInClaim
@@ -1423,6 +1436,15 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         bool full = true;
         for (int step = 0; step < n;) {
             bool overflow = false;
+            // without the look-ahead, this step's LAB is requested before the
+            // claims and barriers, so its latency hides behind them
+            uint32_t pk_cur[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                pk_cur[k] = 0u;
+                if (!kLabAhead)
+                    pk_cur[k] = g_lab[((size_t)min(z + k * K, D - 1) * H + min(y, H - 1)) * W + min(x, W - 1)];
+            }
             if (full) {
                 release();
                 __syncthreads();
@@ -1471,14 +1493,19 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             // the next step's new slabs (and LAB), loaded while this step is evaluated
             uint32_t pk_here[S];
 #pragma unroll
-            for (int k = 0; k < S; ++k) pk_here[k] = pk_next[k];
-            if (step + S < n) {
+            for (int k = 0; k < S; ++k) pk_here[k] = kLabAhead ? pk_next[k] : pk_cur[k];
+            if (kStraight || step + S < n) {
+                // kStraight: issued at the last step too (clamped, unused)
                 const int zn = z + S * K;
 #pragma unroll
                 for (int k = 0; k < S; ++k) load_slab(clampi(zn + (k + 1) * K, 0, D - 1), ids_new[k]);
 #pragma unroll
-                for (int k = 0; k < S; ++k)
-                    if (kLabAhead && mine && step + S + k < n) pk_next[k] = g_lab[((size_t)(zn + k * K) * H + y) * W + x];
+                for (int k = 0; k < S; ++k) {
+                    if (kLabAhead && kStraight)
+                        pk_next[k] = g_lab[((size_t)min(zn + k * K, D - 1) * H + min(y, H - 1)) * W + min(x, W - 1)];
+                    else if (kLabAhead && mine && step + S + k < n)
+                        pk_next[k] = g_lab[((size_t)(zn + k * K) * H + y) * W + x];
+                }
             }
 
             if (mine) {
@@ -1487,7 +1514,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     if (k >= ns) break;
                     const int zk = z + k * K;
                     const size_t v = ((size_t)zk * H + y) * W + x;
-                    const uint32_t pk = kLabAhead ? pk_here[k] : g_lab[v];
+                    const uint32_t pk = pk_here[k];
                     const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u),
                                 l2 = (float)((pk >> 16) & 255u);
                     unsigned long long best = ASV_KEY_INIT;
