@@ -284,13 +284,18 @@ __device__ __forceinline__ int psd3(float m0, float m1, float m2, float m3, floa
     return 1;
 }
 
-// Minimum-norm solution of a rank-deficient system through the eigen-
-// decomposition of N (pseudo-inverse, eigenvalues below 1e-9 of the largest
-// dropped).  Kept out of line: only voxels whose LDL^T pivot collapses take
-// it, and its 6 x 6 work arrays would otherwise weigh on every caller's
-// registers.
-__device__ __noinline__ void pinv_solve(const double* N, const double* c, double* m)
+// Minimum-norm solution of voxel (x, y, t)'s rank-deficient system through
+// the eigendecomposition of N (pseudo-inverse, eigenvalues below 1e-9 of
+// the largest dropped).  Kept out of line, and it rebuilds the normal
+// equations itself: only voxels whose LDL^T pivot collapses take it, and
+// neither its 6 x 6 work arrays nor the address of the caller's system may
+// weigh on every caller (an address passed down would keep the caller's
+// accumulators in scratch memory).
+__device__ __noinline__ void pinv_solve(const float2* __restrict__ fwd, const float2* __restrict__ bwd, int W, int H,
+                                        int D, int x, int y, int t, double* m)
 {
+    double N[21], c[6];
+    normal_equations(fwd, bwd, W, H, D, x, y, t, N, c);
     double a[36], ev[36];
     for (int i = 0; i < 6; ++i)
         for (int j = 0; j < 6; ++j) a[i * 6 + j] = N[tri(i, j)];
@@ -323,7 +328,10 @@ __device__ __forceinline__ int voxel_metric(const float2* __restrict__ fwd, cons
 #pragma unroll
         for (int i = 0; i < 6; ++i) md[i] = 0.0;
     } else if (!ldl_solve(N, c, md)) {
-        pinv_solve(N, c, md);
+        double mp[6];
+        pinv_solve(fwd, bwd, W, H, D, x, y, t, mp);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) md[i] = mp[i];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) m[i] = (float)md[i];
