@@ -1606,7 +1606,7 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x)
 // sums (L and a packed in one word: each field's sum over 64 voxels stays
 // below 2^16).  Only a run's last lane adds it into the block's LDS table,
 // as 32-bit sums (a brick of 64 x kRowBrick x zc voxels, zc <= 8, keeps
-// every slot's sums below 2^32 for W, H, D < 65536); the table goes to the
+// every slot's sums below 2^32 for W, H, D < 2^18); the table goes to the
 // global u64 slots at the end.  Position sums are closed forms: a run of n
 // voxels from x = a to x = e on row (y, z) adds
 //   sum(2x+1) = (e+1)^2 - a^2,  sum(2y+1) = n(2y+1),  sum(2z+1) = n(2z+1).
@@ -1666,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(int W, int H, int D, int A,
                 const uint32_t n = (uint32_t)(lane - h + 1);
                 const uint32_t xa = (uint32_t)(x0 + h), xe1 = (uint32_t)(x + 1);
                 uint32_t vals[7];
-                vals[0] = xe1 * xe1 - xa * xa;
+                vals[0] = n * (xa + xe1);          // (e+1)^2 - a^2 = n (a + e + 1)
                 vals[1] = n * (uint32_t)(2 * y + 1);
                 vals[2] = n * (uint32_t)(2 * z + 1);
                 vals[3] = n;
@@ -2117,7 +2117,9 @@ hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t*
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums)
 {
-    if (d.W > 65535 || d.H > 65535 || d.D > 65535) return hipErrorInvalidValue;   // 32-bit brick sums
+    // 32-bit brick sums: a slot of a 64 x 16 x 8 brick sums at most 8192
+    // voxels' 2c+1 < 2^19 each
+    if (d.W >= (1 << 18) || d.H >= (1 << 18) || d.D >= (1 << 18)) return hipErrorInvalidValue;
     // bricks of 64 columns x kRowBrick rows x zc frames, zc chosen to keep
     // the block count near 2048 on long clips
     const int tiles_x = (int)cdiv(d.W, 64);
