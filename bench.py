@@ -14,6 +14,8 @@ kernel; an iteration's first pass, and its second when sparse, evaluate only
 the seeds' neighbourhoods and are timed separately): 12 algorithmic bytes per
 voxel (int32 label in, packed LAB in, int32 label out) over the pass's
 average launch time from HIP events on the launch stream.
+`quality` is the undersegmentation error and boundary recall of the final
+labels against the generator's ground-truth regions (outside the timing).
 `cpu_baseline` times the CPU oracle on a bounded sample of the same video.
 `served` is the streaming form: the plain per-window loop against the
 served loop with continuous batching over concurrent streams, aggregate
@@ -197,7 +199,7 @@ def main():
 
     seed, W, H, D, ell, vmax, sigma, K, iters = WORKLOADS[args.workload]
     V = W * H * D
-    rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, ell, vmax, sigma)
+    rgb, fwd, bwd, regions = asv.synth_video(seed, W, H, D, ell, vmax, sigma)
     p = asv.default_params(W, H, D, K, lloyd_iters=iters)
     clip = asv.Clip(p)
     clip.upload(rgb, fwd, bwd)
@@ -228,6 +230,10 @@ def main():
     jfa_bytes = 12.0 * V
     achieved = jfa_bytes / (jfa_avg_ms * 1e-3) / 1e9
     steps_list, mult = clip.schedule()
+    # quality of the final labels (GL rows, flipped back to the video's image
+    # rows) against the generator's ground-truth regions
+    from anisotropic_supervoxel_amd import api as asv_api
+    ue, br = asv_api.eval_metrics(np.ascontiguousarray(clip.labels()[:, ::-1, :]), regions)
 
     out = {
         "metric": "Mvoxel-iters/s per GPU (% HBM roofline); p50/p99 window latency, served vs plain",
@@ -268,6 +274,11 @@ def main():
             "seed_and_finalize": round(float(ms[4]), 3),
             "reduction_avg": round(red_avg_ms, 4),
             "reduction_GBps": round(8.0 * V / (red_avg_ms * 1e-3) / 1e9, 1),
+        },
+        "quality": {
+            "undersegmentation_error": round(ue, 5),
+            "boundary_recall": round(br, 5),
+            "against": "ground-truth regions of the synthetic video; final labels before post-processing",
         },
         "cpu_baseline": None,
         "served": None,

@@ -172,6 +172,20 @@ def test_config1_bit_exact(have_gpu):
     _assert_run_equal(got, rgb, fwd, bwd, 500, 10)
 
 
+def test_config1_quality_matches_oracle(have_gpu):
+    """Undersegmentation error and boundary recall against the generator's
+    ground-truth regions, GPU labels vs the oracle's (the labels are
+    bit-exact, so the metrics are equal; the bar is 0.5 %)."""
+    from anisotropic_supervoxel_amd import api
+    rgb, fwd, bwd, regions = asv.synth_video(0, 160, 120, 30, 12, 2.0, 4.0)
+    got = _clip_run(rgb, fwd, bwd, 500, 10, trace=False)
+    want, _, _, _ = orc.run(rgb, fwd, bwd, 500, 10)
+    ue_g, br_g = api.eval_metrics(np.ascontiguousarray(got["labels"][:, ::-1, :]), regions)
+    ue_w, br_w = api.eval_metrics(np.ascontiguousarray(want.reshape(got["labels"].shape)[:, ::-1, :]), regions)
+    assert abs(ue_g - ue_w) <= 0.005 * max(ue_w, 1e-12) and abs(br_g - br_w) <= 0.005 * br_w
+    assert 0.0 < br_g <= 1.0 and ue_g >= 0.0
+
+
 def test_config2_bit_exact(have_gpu):
     """320x240x85, seed 1, 16 ellipses, K=2000 (A=1955), 10 iterations."""
     rgb, fwd, bwd, _ = asv.synth_video(1, 320, 240, 85, 16, 2.0, 4.0)
