@@ -738,6 +738,20 @@ __device__ __forceinline__ unsigned long long dkey(float d, int id)
 {
     return ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)id;
 }
+
+// Minimum of two selection keys in one v_min_f64 (instead of a 64-bit
+// compare and two selects).  With ratio >= 0 a distance is +0 .. +inf or the
+// canonical NaN 0x7FC00000, so a key's high word is below 0x7FF00000: read as
+// a double it is a non-negative number, never NaN, and non-negative doubles
+// (denormals included; the kernels keep fp64 denormals) order exactly like
+// their bit patterns.  The double minimum is therefore the u64 minimum.
+__device__ __forceinline__ unsigned long long key_min(unsigned long long a, unsigned long long b)
+{
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(__longlong_as_double((long long)a)),
+        "v"(__longlong_as_double((long long)b)));
+    return (unsigned long long)__double_as_longlong(r);
+}
 #define ASV_KEY_INIT (((unsigned long long)0x4CBEBC20u) << 32)   /* bits of 1e8f */
 
 // One voxel of a pass, taps read from global memory in the reference's
@@ -1242,6 +1256,10 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // value needs instead of all of them: K = 2 373 -> 355 us, K = 3 508 ->
     // 496; above K = 3 its registers cost a wave per SIMD (K = 5 436 -> 498)
     constexpr bool kStraight = K <= 3;
+    // Records of newly listed ids requested before the new slabs are
+    // converted (their latency overlaps the conversion); above K = 3 the
+    // registers held across the conversion cost a wave per SIMD
+    constexpr bool kStageEarly = K <= 3;
     extern __shared__ uint32_t s_slot[];      // slot_words words: one byte per seed id
     __shared__ uint8_t s_idx[(S + 2) * RA];
     __shared__ float4 s_p[MC];                // (x, y, m0, 2 m1)
@@ -1379,19 +1397,37 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
         };
-        // records of list entries [from, to) from global memory, and the dz
-        // terms of every entry [0, to) for the step's slices z, z + K, ...
-        auto stage = [&](int from, int to, int z) {
-            for (int e = tid; e < to; e += 256) {
+        // records of list entries [from, to), entry e's by thread e (MC <
+        // 256): with kStageEarly requested by stage_load before the new slabs
+        // are converted, else loaded in stage; stage writes them to LDS and
+        // sets the dz terms of every entry [0, to) for the step's slices
+        // z, z + K, ...
+        static_assert(MC < 256, "one list entry per thread");
+        struct Staged { float4 a, b, c; int id; };
+        auto load_rec = [&](int e) {
+            Staged r;
+            r.id = s_list[e];
+            const gptr<const float> q = g_rec + (size_t)r.id * 12;
+            r.a = ld_f4(q);
+            r.b = ld_f4(q + 4);
+            r.c = ld_f4(q + 8);
+            return r;
+        };
+        auto stage_load = [&](int from, int to) {
+            Staged r;                         // read by stage only where loaded
+            if (tid >= from && tid < to) r = load_rec(tid);
+            return r;
+        };
+        auto stage = [&](const Staged& pre, int from, int to, int z) {
+            const int e = tid;
+            if (e < to) {
                 float sz, m2d, m4d, m5, m3;
                 if (e >= from) {
-                    const int id = s_list[e];
-                    const gptr<const float> q = g_rec + (size_t)id * 12;
-                    const float4 a = ld_f4(q), b = ld_f4(q + 4), cc = ld_f4(q + 8);
-                    s_p[e] = make_float4(a.x, a.y, a.w, b.x);
-                    s_z[e] = make_float4(a.z, b.y, b.w, cc.x);
-                    s_c[e] = make_float4(__int_as_float(id), cc.y, cc.z, cc.w);
-                    sz = a.z; m2d = b.y; m4d = b.w; m5 = cc.x; m3 = b.z;
+                    const Staged r = kStageEarly ? pre : load_rec(e);
+                    s_p[e] = make_float4(r.a.x, r.a.y, r.a.w, r.b.x);
+                    s_z[e] = make_float4(r.a.z, r.b.y, r.b.w, r.c.x);
+                    s_c[e] = make_float4(__int_as_float(r.id), r.c.y, r.c.z, r.c.w);
+                    sz = r.a.z; m2d = r.b.y; m4d = r.b.w; m5 = r.c.x; m3 = r.b.z;
                 } else {
                     const float4 zz = s_z[e];
                     sz = zz.x; m2d = zz.y; m4d = zz.z; m5 = zz.w;
@@ -1467,9 +1503,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 known = s_count;
                 overflow = known > MC;
                 if (!overflow) {
+                    Staged rs;
+                    if (kStageEarly) rs = stage_load(0, known);
 #pragma unroll
                     for (int j = 0; j < S + 2; ++j) convert(ids_all[j], j);
-                    stage(0, known, z);
+                    stage(rs, 0, known, z);
                 }
                 __syncthreads();
                 if (!overflow) {
@@ -1487,9 +1525,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     full = true;
                     continue;
                 }
+                Staged rs;
+                if (kStageEarly) rs = stage_load(known, cnt);
 #pragma unroll
                 for (int k = 0; k < S; ++k) convert(ids_new[k], k);
-                stage(known, cnt, z);
+                stage(rs, known, cnt, z);
                 known = cnt;
                 __syncthreads();
                 m[0] = m[S];
@@ -1541,7 +1581,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
                             const unsigned long long kk =
                                 ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.x);
-                            best = kk < best ? kk : best;
+                            best = key_min(kk, best);
                         }
                     } else {
                         const float fz = (float)zk;

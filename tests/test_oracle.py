@@ -251,6 +251,30 @@ def test_jfa_pass_brute_force():
         assert np.array_equal(got, want), k
 
 
+def test_selection_key_min_as_double():
+    """The marching pass takes the minimum of two selection keys
+    (float bits of d << 32 | id) as one fp64 minimum.  For every distance a
+    pass can produce (+0 .. +inf, the canonical NaN 0x7FC00000) and any id,
+    the key read as a double is non-negative and not NaN, and the double
+    minimum picks the same key as the u64 minimum, denormals and the initial
+    key (1e8, 0) included."""
+    rng = np.random.default_rng(12)
+    n = 200000
+    special = np.array([0x0, 0x1, 0x7FFFFF, 0x800000, 0x3F800000, 0x4CBEBC20, 0x4CBEBC21, 0x7F7FFFFF,
+                        0x7F800000, 0x7FC00000], np.uint64)
+    hi = np.concatenate([rng.integers(0, 0x7F800001, n, dtype=np.uint64),
+                         special[rng.integers(0, len(special), n)]])
+    lo = np.concatenate([rng.integers(0, 1 << 14, n, dtype=np.uint64),
+                         rng.integers(0, 4, n, dtype=np.uint64)])
+    keys = (hi << np.uint64(32)) | lo
+    other = np.concatenate([keys[1:], keys[:1]])
+    other[::7] = np.uint64(0x4CBEBC20) << np.uint64(32)
+    a, b = keys.view(np.float64), other.view(np.float64)
+    assert not np.isnan(a).any() and (a >= 0).all()
+    got = np.minimum(a, b).view(np.uint64)
+    assert np.array_equal(got, np.minimum(keys, other))
+
+
 def test_jfa_single_seed_floods_volume():
     D, H, W = 6, 10, 12
     lab = np.zeros((D, H, W), np.uint32)
