@@ -930,9 +930,20 @@ __global__ __launch_bounds__(256) void k_jfa_seeds2(int W, int H, int D, int k1,
 // voxel without a candidate keeps the seed stage's -1 (a seed voxel always
 // has one: its own first-pass label).  A v that occurs in several pairs
 // of its owner's block (small steps, clamped edges) is written with the
-// same value by whichever pair sees the key first.  A tap cell is read once
-// per pair, instead of 27 scattered taps per voxel.
+// same value by whichever pair sees the key first.
+//
+// Layout of the work: waves 0-2 list their axis's step-k1 set (the
+// t-list, at most 2 k1 + 1 <= 33 candidates, one ballot) and then its
+// (t-list index, v) pairs, 64 candidates per ballot, in ascending (t, v)
+// order; in mode 0 wave 3 then reads the tap cells (the product of the
+// three t-lists, 27 for k1 = 1) and their seeds' records into LDS when
+// they are at most kSeed2Taps, so a pair reads its label and record from
+// LDS and only the voxel's LAB from global memory.  Each thread takes
+// three pairs at a time and issues their loads before it uses them.
 constexpr int kSeed2Pairs = (kSeedPassMaxStep + 1) * (kSeedPassMaxStep + 1);
+constexpr int kSeed2Axis = 2 * kSeedPassMaxStep + 1;      // t-list candidates per axis
+constexpr int kSeed2Taps = 64;
+constexpr int kSeed2Batch = 3;
 
 __global__ __launch_bounds__(256) void k_jfa_seeds2_scatter(int W, int H, int D, int k1, int k2, int mode,
                                                             const int4* __restrict__ pos,
@@ -941,47 +952,120 @@ __global__ __launch_bounds__(256) void k_jfa_seeds2_scatter(int W, int H, int D,
                                                             const SeedRec* __restrict__ rec, float ratio,
                                                             unsigned long long* __restrict__ keys)
 {
-    __shared__ short s_t[3][kSeed2Pairs];
-    __shared__ short s_v[3][kSeed2Pairs];
-    __shared__ int s_n[3];
+    __shared__ short s_t[3][kSeed2Pairs];     // pair: index into the axis's t-list
+    __shared__ short s_v[3][kSeed2Pairs];     // pair: voxel coordinate
+    __shared__ short s_a[3][kSeed2Axis];      // t-list: tap coordinates
+    __shared__ int s_n[3], s_m[3];            // pairs and t-list length per axis
+    __shared__ int s_tap[kSeed2Taps];         // staged tap labels
+    __shared__ float4 s_r[3][kSeed2Taps];     // and their seeds' records
     const int4 p = pos[blockIdx.x];
-    if (threadIdx.x < 3) {
-        const int ax = threadIdx.x;
-        const int pa = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
-        const int n = ax == 0 ? W : (ax == 1 ? H : D);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const int ax = wave < 3 ? wave : 0;
+    const int pa = ax == 0 ? p.x : (ax == 1 ? p.y : p.z);
+    const int na = ax == 0 ? W : (ax == 1 ? H : D);
+    if (wave < 3) {
+        const int a = pa - k1 + lane;
+        const bool hit = lane <= 2 * k1 && a >= 0 && a < na && axis_hit(a, pa, na, k1);
+        const unsigned long long b = __ballot(hit);
+        if (hit) s_a[ax][__popcll(b & below)] = (short)a;
+        if (lane == 0) s_m[ax] = __popcll(b);
+    }
+    __syncthreads();
+    const int mx = s_m[0], my = s_m[1], mz = s_m[2];
+    const bool staged = mode == 0 && mx * my * mz <= kSeed2Taps;
+    if (wave < 3) {
+        const int span = 2 * k2 + 1, cand = s_m[ax] * span;
         int cnt = 0;
-        for (int a = max(0, pa - k1); a <= min(n - 1, pa + k1); ++a) {
-            if (!axis_hit(a, pa, n, k1)) continue;
-            for (int c = max(0, a - k2); c <= min(n - 1, a + k2); ++c)
-                if (axis_hit(c, a, n, k2)) {
-                    s_t[ax][cnt] = (short)a;
-                    s_v[ax][cnt] = (short)c;
-                    ++cnt;
-                }
+        for (int q0 = 0; q0 < cand; q0 += 64) {
+            const int q = q0 + lane;
+            const int i = q / span;
+            bool hit = q < cand;
+            int c = 0;
+            if (hit) {
+                const int a = s_a[ax][i];
+                c = a - k2 + (q - i * span);
+                hit = c >= 0 && c < na && axis_hit(c, a, na, k2);
+            }
+            const unsigned long long b = __ballot(hit);
+            if (hit) {
+                const int w = cnt + __popcll(b & below);
+                s_t[ax][w] = (short)i;
+                s_v[ax][w] = (short)c;
+            }
+            cnt += __popcll(b);
         }
-        s_n[ax] = cnt;
+        if (lane == 0) s_n[ax] = cnt;
+    } else if (staged && lane < mx * my * mz) {
+        const int x = s_a[0][lane % mx], y = s_a[1][(lane / mx) % my], z = s_a[2][lane / (mx * my)];
+        const int id = in[((size_t)z * H + y) * W + x];
+        s_tap[lane] = id;
+        if (id >= 0) {
+            const SeedRec r = rec[id];
+            s_r[0][lane] = r.a;
+            s_r[1][lane] = r.b;
+            s_r[2][lane] = r.c;
+        }
     }
     __syncthreads();
     const int nx = s_n[0], ny = s_n[1], nz = s_n[2];
     const int total = nx * ny * nz;
-    for (int j = threadIdx.x; j < total; j += 256) {
-        const int ix = j % nx, iy = (j / nx) % ny, iz = j / (nx * ny);
-        const size_t v = ((size_t)s_v[2][iz] * H + s_v[1][iy]) * W + s_v[0][ix];
+    for (int j0 = 0; j0 < total; j0 += 256 * kSeed2Batch) {
+        size_t v[kSeed2Batch];
+        int ix[kSeed2Batch], iy[kSeed2Batch], iz[kSeed2Batch];
+        bool ok[kSeed2Batch];
+#pragma unroll
+        for (int u = 0; u < kSeed2Batch; ++u) {
+            const int j = j0 + u * 256 + tid;
+            ok[u] = j < total;
+            const int jj = ok[u] ? j : 0;
+            ix[u] = jj % nx;
+            iy[u] = (jj / nx) % ny;
+            iz[u] = jj / (nx * ny);
+            v[u] = ((size_t)s_v[2][iz[u]] * H + s_v[1][iy[u]]) * W + s_v[0][ix[u]];
+        }
         if (mode == 0) {
-            const int32_t id = in[((size_t)s_t[2][iz] * H + s_t[1][iy]) * W + s_t[0][ix]];
-            if (id < 0) continue;
-            const uint32_t pk = lab[v];
-            const SeedRec r = rec[id];
-            const float d = cand_dist(r.a, r.b, r.c, (float)s_v[0][ix], (float)s_v[1][iy], (float)s_v[2][iz],
-                                      (float)(pk & 255u), (float)((pk >> 8) & 255u), (float)((pk >> 16) & 255u),
-                                      ratio);
-            const unsigned long long kk = dkey(d, id);
-            if (kk < ASV_KEY_INIT) atomicMin(keys + v, kk);
+            int id[kSeed2Batch], tap[kSeed2Batch];
+            uint32_t pk[kSeed2Batch];
+#pragma unroll
+            for (int u = 0; u < kSeed2Batch; ++u) {
+                const int tx = s_t[0][ix[u]], ty = s_t[1][iy[u]], tz = s_t[2][iz[u]];
+                tap[u] = tx + mx * (ty + my * tz);
+                id[u] = staged ? s_tap[tap[u]]
+                               : in[((size_t)s_a[2][tz] * H + s_a[1][ty]) * W + s_a[0][tx]];
+                if (!ok[u]) id[u] = -1;
+                pk[u] = lab[id[u] >= 0 ? v[u] : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < kSeed2Batch; ++u) {
+                if (id[u] < 0) continue;
+                float4 ra, rb, rc;
+                if (staged) {
+                    ra = s_r[0][tap[u]];
+                    rb = s_r[1][tap[u]];
+                    rc = s_r[2][tap[u]];
+                } else {
+                    const SeedRec r = rec[id[u]];
+                    ra = r.a;
+                    rb = r.b;
+                    rc = r.c;
+                }
+                const float d = cand_dist(ra, rb, rc, (float)s_v[0][ix[u]], (float)s_v[1][iy[u]],
+                                          (float)s_v[2][iz[u]], (float)(pk[u] & 255u), (float)((pk[u] >> 8) & 255u),
+                                          (float)((pk[u] >> 16) & 255u), ratio);
+                const unsigned long long kk = dkey(d, id[u]);
+                if (kk < ASV_KEY_INIT) atomicMin(keys + v[u], kk);
+            }
         } else {
-            const unsigned long long kk = keys[v];
-            if (kk < ASV_KEY_INIT && (unsigned)(kk & 0xffffffffull) == (unsigned)blockIdx.x) {
-                out[v] = (int32_t)blockIdx.x;
-                keys[v] = ASV_KEY_INIT;
+            unsigned long long kk[kSeed2Batch];
+#pragma unroll
+            for (int u = 0; u < kSeed2Batch; ++u) kk[u] = keys[ok[u] ? v[u] : 0];
+#pragma unroll
+            for (int u = 0; u < kSeed2Batch; ++u) {
+                if (ok[u] && kk[u] < ASV_KEY_INIT && (unsigned)(kk[u] & 0xffffffffull) == (unsigned)blockIdx.x) {
+                    out[v[u]] = (int32_t)blockIdx.x;
+                    keys[v[u]] = ASV_KEY_INIT;
+                }
             }
         }
     }

@@ -141,6 +141,9 @@ def test_seed_metric_fallbacks_bit_exact(have_gpu, gate):
     (13, (40, 80, 96), 20, 3, {}),
     (14, (30, 64, 64), 12, 3, {"tie_rule": 1}),
     (15, (16, 40, 200), 30, 3, {}),
+    # halving schedule, first steps 12 and 6: the second pass's 13^3 tap
+    # cells are read from global memory instead of LDS
+    (16, (40, 200, 24), 20, 3, {"schedule": 1}),
 ])
 def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
     D, H, W = shape
