@@ -1742,7 +1742,7 @@ __global__ __launch_bounds__(256) void k_reduce_rows(int W, int H, int D, int A,
                                                      unsigned long long* __restrict__ sums)
 {
     __shared__ int s_key[kRedSlots];
-    __shared__ uint32_t s_acc[kRedSlots][8];
+    __shared__ uint32_t s_acc[kRedSlots][9];  // rows of 9 words: the slots of a wave's run tails fall in different banks
     const int tid = threadIdx.x;
     s_key[tid] = kEmpty;
 #pragma unroll
