@@ -180,7 +180,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=70)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-streams", type=int, default=24)
-    ap.add_argument("--served-active", default="4,8", help="comma-separated concurrent-stream limits")
+    ap.add_argument("--served-active", default="4,6,8", help="comma-separated concurrent-stream limits")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-clips", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=2)
