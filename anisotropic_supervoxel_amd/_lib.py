@@ -46,6 +46,7 @@ SIGNATURES = {
     "asv_download_labels": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_sums": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_enable_trace": (ctypes.c_int, [_vp, _i32]),
+    "asv_force_exact_clamp": (ctypes.c_int, [_vp, _i32]),
     "asv_download_trace": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_lab": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_metric": (ctypes.c_int, [_vp, _vp, _vp]),

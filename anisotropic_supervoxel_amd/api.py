@@ -276,6 +276,10 @@ class Clip:
     def enable_trace(self, on=True):
         check(self.lib.asv_enable_trace(self.h, 1 if on else 0))
 
+    def force_exact_clamp(self, on=True):
+        """Every later run takes the marching pass's exact per-tap path."""
+        check(self.lib.asv_force_exact_clamp(self.h, 1 if on else 0))
+
     # -- outputs -------------------------------------------------------------
     def labels(self):
         out = np.empty(self.shape, np.int32)

@@ -74,6 +74,7 @@ JfaClip descriptor(const asv_ctx* c, int p)
     d.tiles_x = (c->d.W + 15) / 16;
     d.tiles_y = (c->d.H + 15) / 16;
     d.ratio = c->mult[(size_t)p] * c->ratio;
+    d.unsafe = c->unsafe;
     return d;
 }
 
@@ -138,7 +139,7 @@ int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
             BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
                                    c->keys));
             break;
-        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule)); break;
+        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe)); break;
         }
     }
     return ASV_OK;

@@ -6,6 +6,7 @@
 #include "asv_host_internal.h"
 #include "asv_ctx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -106,7 +107,7 @@ void schedule_list(int mode, int s, int W, std::vector<int>& steps, std::vector<
     }
 }
 
-// sRGB linearisation table (RGB2LAB.fs:77-90): double pow rounded to float.
+// sRGB linearisation table (RGB2LAB.fs:27-40): double pow rounded to float.
 void lin_table(float* lut)
 {
     for (int c = 0; c < 256; ++c) {
@@ -134,7 +135,7 @@ bool valid_params(const asv_params* p)
     if (p->width < 2 || p->height < 2 || p->depth < 1) return false;
     if ((int64_t)p->width * p->height * p->depth > (int64_t)1 << 30) return false;
     if (p->seeds_num < 1 || p->lloyd_iters < 1) return false;
-    if (!(p->compactness > 0.0f)) return false;
+    if (!(p->compactness > 0.0f) || !std::isfinite(p->compactness)) return false;
     if (p->tie_rule != ASV_TIE_LOWEST_ID && p->tie_rule != ASV_TIE_SCAN_ORDER) return false;
     if (p->schedule != ASV_SCHED_REFERENCE && p->schedule != ASV_SCHED_HALVING_PLUS_ONE) return false;
     if (p->seed_init != ASV_INIT_GRID && p->seed_init != ASV_INIT_DENSITY) return false;
@@ -150,9 +151,10 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->seed_own, c->seed_need,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys};
+                    c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys, c->unsafe};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    if (c->sums_host) (void)hipHostFree(c->sums_host);
 }
 
 // Whether a context keeps the keys of the scatter-form second pass: the
@@ -287,6 +289,14 @@ int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out
     if (e == hipSuccess) e = dalloc(&c->seed_need, (size_t)Acap, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[0], (size_t)(Acap + 1) * kSumWords, &tb);
     if (e == hipSuccess) e = dalloc(&c->sums[1], (size_t)(Acap + 1) * kSumWords, &tb);
+    if (e == hipSuccess) e = dalloc(&c->unsafe, 1, &tb);
+    if (e == hipSuccess) e = launch_zero_i32(nullptr, c->unsafe, 1);
+    // page-locked landing buffers of the small device-to-host copies (sums,
+    // checksum): explicit device-to-host transfers into memory the context
+    // owns, whatever buffer the caller passes
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&c->sums_host, sizeof(unsigned long long) * ((size_t)(Acap + 1) * kSumWords + 1),
+                          hipHostMallocDefault);
     if (e == hipSuccess && wants_keys(p, c->steps, A)) {
         e = dalloc(&c->keys, (size_t)V, &tb);
         if (e == hipSuccess) e = launch_fill_keys(nullptr, c->keys, V);
@@ -342,10 +352,15 @@ int density_reseed(asv_ctx* c, hipStream_t st)
     for (int i = 0; i < wN; ++i) { box(px, wN, W, i, &lo, &hi); for (int x = std::max(lo, 0); x <= std::min(hi, W - 1); ++x) mx[x] = i; }
     for (int j = 0; j < hN; ++j) { box(py, hN, H, j, &lo, &hi); for (int y = std::max(lo, 0); y <= std::min(hi, H - 1); ++y) my[y] = j; }
     for (int t = 0; t < dN; ++t) { box(pz, dN, D, t, &lo, &hi); for (int z = std::max(lo, 0); z <= std::min(hi, D - 1); ++z) mz[z] = t; }
-    const int nslot = (int)(float)(_Float16)(float)(A - 1) + 1;     // ids pass through R16F
+    // Ids pass through R16F, so the highest slot a voxel can address is the
+    // float16 rounding of A - 1, which may lie below A - 1 (A = 2146: 2145
+    // rounds to 2144).  The reference reads all A texels (Image.cpp:524-552);
+    // a slot no voxel addresses holds only the clear colour, so the host
+    // table covers max(nslot, A) slots, zero beyond the device's nslot.
+    const int nslot = (int)(float)(_Float16)(float)(A - 1) + 1;
     int* d_maps = nullptr;
     unsigned long long* d_acc = nullptr;
-    std::vector<unsigned long long> acc((size_t)nslot);
+    std::vector<unsigned long long> acc((size_t)std::max(nslot, A), 0ull);
     hipError_t e = hipMalloc((void**)&d_maps, sizeof(int) * maps.size());
     if (e == hipSuccess) e = hipMalloc((void**)&d_acc, sizeof(unsigned long long) * (size_t)nslot);
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice, st);
@@ -568,6 +583,8 @@ int asv_prepare(asv_ctx* c, void* stream)
     if (r) return r;
     hipStream_t st = S(stream);
     HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->lut, c->lab));
+    // the seed stages of the run raise it when a metric needs the exact clamp
+    HIP_TRY(launch_fill(st, c->unsafe, 1, c->force_exact ? 1 : 0));
     if (c->p.seed_init == ASV_INIT_DENSITY && c->gdims[0] > 0 && (r = density_reseed(c, st))) return r;
     c->prepared = true;
     return ASV_OK;
@@ -591,7 +608,7 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
     float* tr = c->trace_on && c->trace ? c->trace + (size_t)(iter - 1) * c->d.A * 12 : nullptr;
     int gate = (iter >= 2 && c->p.angle_gate) ? 1 : 0;
     HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->fwd, c->bwd, gate, c->rec, c->seed_own, c->seed_need,
-                               tr));
+                               tr, c->unsafe));
     HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
     HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
     c->fresh = true;
@@ -619,7 +636,7 @@ hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
     case 2:
         return launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], c->steps[1], ratio,
                                  c->p.tie_rule, c->keys);
-    default: return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule);
+    default: return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule, c->unsafe);
     }
 }
 
@@ -761,12 +778,12 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
     if (checksum) {
         if (!c->ck_word) HIP_TRY(hipMalloc((void**)&c->ck_word, sizeof(unsigned long long)));
         unsigned long long* ck = c->ck_word;
-        unsigned long long host = 0ull;
+        unsigned long long* host = c->sums_host + (size_t)(c->seed_cap + 1) * kSumWords;
         HIP_TRY(launch_zero_u64(st, reinterpret_cast<uint64_t*>(ck), 1));
         HIP_TRY(launch_checksum(st, scratch, n, index0, ck));
-        HIP_TRY(hipMemcpyAsync(&host, ck, sizeof(host), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(host, ck, sizeof(*host), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        *checksum = host;
+        *checksum = *host;
         return ASV_OK;
     }
     HIP_TRY(hipStreamSynchronize(st));
@@ -964,9 +981,17 @@ int asv_download_sums(asv_ctx* c, int64_t* sums, void* stream)
 {
     if (!c || !sums) return fail(ASV_ERR_ARG, "null argument");
     if (c->last_iter < 1) return fail(ASV_ERR_STATE, "no run has completed");
-    HIP_TRY(hipMemcpyAsync(sums, c->sums[c->last_iter & 1], (size_t)(c->d.A + 1) * kSumWords * 8,
-                           hipMemcpyDefault, S(stream)));
+    const size_t bytes = (size_t)(c->d.A + 1) * kSumWords * 8;
+    HIP_TRY(hipMemcpyAsync(c->sums_host, c->sums[c->last_iter & 1], bytes, hipMemcpyDeviceToHost, S(stream)));
     HIP_TRY(hipStreamSynchronize(S(stream)));
+    std::memcpy(sums, c->sums_host, bytes);
+    return ASV_OK;
+}
+
+int asv_force_exact_clamp(asv_ctx* c, int32_t on)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    c->force_exact = on != 0;
     return ASV_OK;
 }
 

@@ -45,6 +45,12 @@ struct asv_ctx {
     int64_t post_cap = 0;
     unsigned long long* ck_word = nullptr;
     unsigned long long* keys = nullptr;     // V keys for the scatter form of the sparse second pass
+    int* unsafe = nullptr;                  // raised when a seed metric needs the exact clamp (JfaClip::unsafe)
+    bool force_exact = false;               // asv_force_exact_clamp: raise it at every prepare
+    // page-locked host words: (seed_cap + 1) x kSumWords sums, then the
+    // checksum word; every device-to-host copy of sums and checksums lands
+    // here first (one explicit transfer into memory the context owns)
+    unsigned long long* sums_host = nullptr;
 
     int seed_cap = 0;       // seeds the buffers hold (asv_reseed)
     int depth_cap = 0;      // frames the volume buffers hold

@@ -29,6 +29,11 @@ struct JfaClip {
     int tiles_x, tiles_y, seg, nseg;
     int item_begin;
     float ratio;
+    // Raised by the seed stage when some seed's metric could make a
+    // quadratic form NaN (launch_seed_metric); the candidate rounds clamp
+    // with one integer maximum, exact only without NaN, so the clip's
+    // columns are then evaluated tap by tap with the exact clamp.
+    const int* unsafe;
 };
 
 // Per-slot centroid sums: sum(2x+1), sum(2y+1), sum(2z+1), n, sum L, a, b, pad.
@@ -56,15 +61,18 @@ hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, con
                             int4* pos, float4* slab);
 // Seed metrics and records: the metric at each seed's voxel, ring fallbacks
 // for the seeds where it is not PSD (own: 2 float4 per seed, need: 1 int
-// per seed, scratch of the launch).
+// per seed, scratch of the launch).  A metric with an entry that is not
+// finite or exceeds 2^60 in magnitude sets *unsafe to 1 (the marching pass
+// then evaluates tap by tap, see JfaClip::unsafe); the caller clears it.
 hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab, const float2* fwd,
-                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter);
+                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter,
+                              int* unsafe);
 hipError_t launch_fill(hipStream_t st, int32_t* p, int64_t n, int32_t value);
 hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n);
 hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
 hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
-                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule);
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe);
 // The first pass of an iteration when `in` holds only the seeds scattered
 // by the seed stage at `pos` (every other voxel -1): fills `out` with -1
 // and evaluates only the voxels that have a seed among their taps, at most

@@ -9,11 +9,11 @@
 //
 // Reference passes replaced (paths relative to the reference root):
 //   k_prepare             shaders/RGB2LAB.fs:11-79 (blur + flip of Render.cpp:354-359)
-//   k_seed_init           shaders/draw_seeds.vs:20-35 (colour at the seed voxel)
+//   k_seed_init           shaders/draw_seeds.vs:20-34 (colour at the seed voxel)
 //   k_seed_metric(_rings) Image.cpp:689-872 and :1351-1534, with matrix_slover
 //                         (Image.cpp:874-1009) evaluated at the seeds and rings
 //   k_fill + k_place      draw_seeds / draw_next_seeds (clear -1, later id wins)
-//   k_jfa                 shaders/flood.gs:53-209
+//   k_jfa                 shaders/flood.gs:26-182
 //   k_reduce_rows         shaders/centroids.vs:19-53 with additive blending
 //   k_finalize            shaders/aver_exception.vs:22-81
 
@@ -563,9 +563,25 @@ __device__ void nearest_psd(const float in[6], float out[6])
     out[3] = (float)r[4]; out[4] = (float)r[5]; out[5] = (float)r[8];
 }
 
-__device__ __forceinline__ void write_seed(int i, int4 p, float4 c, const float m[6], SeedRec* __restrict__ rec,
-                                           float* __restrict__ trace_iter)
+// Whether no quadratic form with this metric can be NaN: every entry finite
+// and at most 2^60 in magnitude.  Voxel offsets stay below 2^29 (volumes
+// hold at most 2^30 voxels), so every product and sum of the distance then
+// stays below 2^124, finite, and the rounds' one-instruction clamp (nonneg)
+// is exact.
+__device__ __forceinline__ bool metric_safe(const float m[6])
 {
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ok = ok && fabsf(m[k]) <= 1.152921504606846976e18f;   // NaN fails
+    return ok;
+}
+
+// Writes seed i's record; a metric that fails metric_safe raises *unsafe,
+// which sends the run's marching passes down the exact per-tap path.
+__device__ __forceinline__ void write_seed(int i, int4 p, float4 c, const float m[6], SeedRec* __restrict__ rec,
+                                           float* __restrict__ trace_iter, int* __restrict__ unsafe)
+{
+    if (!metric_safe(m)) atomicOr(unsafe, 1);
     SeedRec r;
     r.a = make_float4((float)p.x, (float)p.y, (float)p.z, m[0]);
     r.b = make_float4(2.0f * m[1], 2.0f * m[2], m[3], 2.0f * m[4]);
@@ -587,7 +603,7 @@ __global__ __launch_bounds__(64) void k_seed_metric(int W, int H, int D, int A, 
                                                     const float4* __restrict__ slab, const float2* __restrict__ fwd,
                                                     const float2* __restrict__ bwd, SeedRec* __restrict__ rec,
                                                     float4* __restrict__ own, int* __restrict__ need,
-                                                    float* __restrict__ trace_iter)
+                                                    float* __restrict__ trace_iter, int* __restrict__ unsafe)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A) return;
@@ -596,7 +612,7 @@ __global__ __launch_bounds__(64) void k_seed_metric(int W, int H, int D, int A, 
     const int ok = voxel_metric(fwd, bwd, W, H, D, p.x, p.y, p.z, m);
     need[i] = ok ? 0 : 1;
     if (ok) {
-        write_seed(i, p, slab[i], m, rec, trace_iter);
+        write_seed(i, p, slab[i], m, rec, trace_iter, unsafe);
     } else {
         own[2 * i] = make_float4(m[0], m[1], m[2], m[3]);
         own[2 * i + 1] = make_float4(m[4], m[5], 0.0f, 0.0f);
@@ -614,7 +630,8 @@ __global__ __launch_bounds__(64) void k_seed_metric_rings(int W, int H, int D, c
                                                           const float2* __restrict__ fwd,
                                                           const float2* __restrict__ bwd, int gate,
                                                           SeedRec* __restrict__ rec, const float4* __restrict__ own,
-                                                          const int* __restrict__ need, float* __restrict__ trace_iter)
+                                                          const int* __restrict__ need, float* __restrict__ trace_iter,
+                                                          int* __restrict__ unsafe)
 {
     const int i = blockIdx.x;
     if (!need[i]) return;
@@ -661,7 +678,7 @@ __global__ __launch_bounds__(64) void k_seed_metric_rings(int W, int H, int D, c
         const float self[6] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y};
         nearest_psd(self, m);
     }
-    write_seed(i, p, slab[i], m, rec, trace_iter);
+    write_seed(i, p, slab[i], m, rec, trace_iter, unsafe);
 }
 
 // ---------------------------------------------------------------------------
@@ -701,19 +718,28 @@ __global__ void k_place(int W, int H, int A, const int4* __restrict__ pos, int32
 }
 
 // ---------------------------------------------------------------------------
-// JFA pass (flood.gs:53-209)
+// JFA pass (flood.gs:26-182)
 // ---------------------------------------------------------------------------
 
-// The non-negative part of a quadratic form value: a value with its sign
-// bit set (negative, -0) becomes +0, any other value (NaN included) passes.
+// The non-negative part of a quadratic form value that is not NaN: a value
+// with its sign bit set (negative, -0) becomes +0.  One integer maximum; the
+// candidate rounds of the marching pass use it where the seed stage has
+// checked that no quadratic form can be NaN (metric_safe).
 __device__ __forceinline__ float nonneg(float q) { return __int_as_float(max(__float_as_int(q), 0)); }
+
+// The distance every other path uses.  flood.gs:90-95 clamps a negative
+// quadratic form to 0 (`if(curr_pos_dist<0)`), which leaves a NaN as it
+// is, and the NaN distance then loses `curr_dist<min_dist`: here a NaN
+// quadratic form (of either sign) gives a +inf distance, which every
+// selection rejects as it rejects any distance >= 1e8.
+constexpr uint32_t kRejectBits = 0x7F800000u;   // +inf
 
 // Distance of voxel (fx, fy, fz) with colour (l0, l1, l2) to one seed record
 // (a = (x, y, z, m0), b = (2m1, 2m2, m3, 2m4), c = (m5, L, a, b)), in the
 // fixed order of the specification:
 //   u1 = dz*2m2,  u2 = dz*2m4,  u3 = (dz*dz)*m5     (rounded products)
 //   t1 = fma(dx, m0, fma(dy, 2m1, u1)),  t2 = fma(dy, m3, u2)
-//   q  = nonneg(fma(dx, t1, fma(dy, t2, u3)))
+//   q  = fma(dx, t1, fma(dy, t2, u3)): NaN -> +inf distance, else nonneg
 //   d  = fma(ratio, q, fma(c2, c2, fma(c1, c1, c0*c0)))
 // The u terms depend on the seed and the slice only, so a block working on
 // one slice can compute them once per seed.
@@ -724,7 +750,9 @@ __device__ __forceinline__ float cand_dist(const float4 a, const float4 b, const
     const float u1 = dz * b.y, u2 = dz * b.w, u3 = (dz * dz) * c.x;
     const float t1 = __fmaf_rn(dx, a.w, __fmaf_rn(dy, b.x, u1));
     const float t2 = __fmaf_rn(dy, b.z, u2);
-    const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u3)));
+    const float qr = __fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u3));
+    // a select, not a branch: +inf makes d = +inf (ratio > 0)
+    const float q = qr != qr ? __uint_as_float(kRejectBits) : nonneg(qr);
     const float c0 = l0 - c.y, c1 = l1 - c.z, c2 = l2 - c.w;
     const float cd = __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0));
     return __fmaf_rn(ratio, q, cd);
@@ -733,7 +761,7 @@ __device__ __forceinline__ float cand_dist(const float4 a, const float4 b, const
 // Selection key for the lowest-id rule: distances are >= +0 (or NaN), so the
 // float bits order like the values; (d, id) lexicographic minimum = u64 min.
 // The initial key (1e8, 0) rejects d >= 1e8 exactly as `d < min_dist` with
-// min_dist = 1e8 does (flood.gs:90, :122).
+// min_dist = 1e8 does (flood.gs:63, :95).
 __device__ __forceinline__ unsigned long long dkey(float d, int id)
 {
     return ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)id;
@@ -1418,6 +1446,10 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const int x = x0 + lx, y = y0 + ly;
         const bool mine = x < W && y < H;
         const float fx = (float)x, fy = (float)y;
+        // A clip whose seed stage raised `unsafe` (a metric that could make a
+        // quadratic form NaN) takes the per-tap path at every step, which
+        // clamps with cand_dist's exact rule (the flag is uniform per item)
+        const bool exact = c.unsafe && *(gptr<const int>)c.unsafe;
 
         auto load_slab = [&](int gz, int* ids) {
             const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
@@ -1585,7 +1617,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 for (int j = 0; j < S + 2; ++j) claim(ids_all[j]);
                 __syncthreads();
                 known = s_count;
-                overflow = known > MC;
+                overflow = known > MC || exact;
                 if (!overflow) {
                     Staged rs;
                     if (kStageEarly) rs = stage_load(0, known);
@@ -1668,19 +1700,21 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             best = key_min(kk, best);
                         }
                     } else {
+                        // per tap from global labels and records, the tap
+                        // coordinates formed as the loop goes (fewer live
+                        // registers than three precomputed lists)
                         const float fz = (float)zk;
-                        const int xs3[3] = {clampi(x - K, 0, W - 1), x, clampi(x + K, 0, W - 1)};
-                        const int ys3[3] = {clampi(y - K, 0, H - 1), y, clampi(y + K, 0, H - 1)};
-                        const int zs3[3] = {zk, clampi(zk - K, 0, D - 1), clampi(zk + K, 0, D - 1)};
                         int seen0 = -1, seen1 = -1;
 #pragma unroll 1
-                        for (int sl = 0; sl < 3; ++sl)
+                        for (int sl = 0; sl < 3; ++sl) {
+                            const int zz = sl == 0 ? zk : clampi(zk + (sl == 1 ? -K : K), 0, D - 1);
 #pragma unroll 1
-                            for (int rr = 0; rr < 3; ++rr) {
-                                const gptr<const int32_t> row = g_in + ((size_t)zs3[sl] * H + ys3[rr]) * W;
+                            for (int rr = -1; rr <= 1; ++rr) {
+                                const gptr<const int32_t> row =
+                                    g_in + ((size_t)zz * H + clampi(y + rr * K, 0, H - 1)) * W;
 #pragma unroll 1
-                                for (int cc = 0; cc < 3; ++cc) {
-                                    const int id = row[xs3[cc]];
+                                for (int cc = -1; cc <= 1; ++cc) {
+                                    const int id = row[clampi(x + cc * K, 0, W - 1)];
                                     if (id < 0 || id == seen0 || id == seen1) continue;
                                     seen1 = seen0;
                                     seen0 = id;
@@ -1691,6 +1725,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                                     best = kk < best ? kk : best;
                                 }
                             }
+                        }
                     }
                     g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
                 }
@@ -1919,13 +1954,15 @@ hipError_t launch_seed_init(hipStream_t st, const Dims& d, const int4* grid, con
 }
 
 hipError_t launch_seed_metric(hipStream_t st, const Dims& d, const int4* pos, const float4* slab, const float2* fwd,
-                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter)
+                              const float2* bwd, int gate, SeedRec* rec, float4* own, int* need, float* trace_iter,
+                              int* unsafe)
 {
     if (d.A <= 0) return hipSuccess;
+    if (!unsafe) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_seed_metric, dim3(cdiv(d.A, 64)), dim3(64), 0, st, d.W, d.H, d.D, d.A, pos, slab, fwd, bwd,
-                       rec, own, need, trace_iter);
+                       rec, own, need, trace_iter, unsafe);
     hipLaunchKernelGGL(k_seed_metric_rings, dim3((unsigned)d.A), dim3(64), 0, st, d.W, d.H, d.D, pos, slab, fwd,
-                       bwd, gate, rec, own, need, trace_iter);
+                       bwd, gate, rec, own, need, trace_iter, unsafe);
     return hipGetLastError();
 }
 
@@ -2140,11 +2177,11 @@ hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, 
 
 template <int K>
 static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                              const SeedRec* rec, float ratio)
+                              const SeedRec* rec, float ratio, const int* unsafe)
 {
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     if (d.A <= kMarchMaxIds) {
-        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio};
+        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe};
         int nitems = 0, dyn = 0;
         const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
         if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn);
@@ -2205,27 +2242,27 @@ hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n)
 }
 
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
-                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule)
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe)
 {
     int64_t ntiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile) * d.D;
     if (tie_rule == 0 && ntiles < (1ll << 31)) {
         switch (k) {
-        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio);
-        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio);
-        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio);
-        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio);
-        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio);
-        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio);
-        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio);
-        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio);
-        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio);
-        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio);
-        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio);
-        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio);
-        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio);
-        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio);
-        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio);
-        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio);
+        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio, unsafe);
         default: break;
         }
     }

@@ -213,6 +213,15 @@ int asv_connectivity_labels(const int32_t* in, int32_t* out, int32_t width, int3
 int asv_host_alloc(int64_t bytes, void** out);
 int asv_host_free(void* p);
 
+/* The JFA distance clamps a negative quadratic form to 0 and rejects a NaN
+ * one, as flood.gs:90-95 leaves a NaN to lose `curr_dist<min_dist`.  The
+ * marching pass's candidate rounds clamp with one integer maximum, exact
+ * while no form can be NaN; when a seed metric has an entry that is not
+ * finite or exceeds 2^60 in magnitude, the seed stage switches the run's
+ * marching passes to an exact per-tap path.  on = 1 takes that path in
+ * every later run of the context (for tests and tools); no counterpart in
+ * the reference. */
+int asv_force_exact_clamp(asv_ctx* ctx, int32_t on);
 /* Per-iteration seed records, when enabled before the run:
  * lloyd_iters x A x 12 float (x, y, z, L, a, b, m0..m5). */
 int asv_enable_trace(asv_ctx* ctx, int32_t on);

@@ -32,10 +32,24 @@
  *   orc_grid_seeds    placed-seed counts 468 / 1955 / 9720 for the three
  *                     plain configurations and perfect-cube truncation.
  *   orc_schedule      step lists 1,5,3,2,1,2,1 / 1,7,4,2,1,2,1 / 1,9,5,3,2,1,2,1.
+ *   seed_metric       constructed fields (tests/test_oracle_pins.py): own PSD
+ *                     metric; one PSD 1-ring neighbour exactly; float means
+ *                     summed in the ring order of Image.cpp:701-702 and
+ *                     :766-767 (values whose sum depends on the order);
+ *                     empty 1-ring -> 2-ring; out-of-volume neighbours; both
+ *                     rings empty -> nearest PSD; the angle gate on and off,
+ *                     zero flow and the last frame.
+ *   angle_gate        the reference expression (float numerator, double
+ *                     norms, float acos correctly rounded, PI = 3.141592653,
+ *                     theta < 90) over every float cosine near the threshold
+ *                     and random flows.
  *   orc_jfa_pass      brute force over the 27 clamped taps in numpy on tiny
- *                     volumes; one-seed flood; invariants.
+ *                     volumes, both tie rules (the scan-order rule with many
+ *                     exact ties); one-seed flood; invariants; NaN and
+ *                     negative quadratic forms.
  *   orc_run           exact-sum identities (sum of counts = V), constructed
- *                     dead seed, collision rule.
+ *                     dead seed, two seeds on one voxel (the higher id wins,
+ *                     the loser takes the neighbour-slot rule).
  * Eigen's SVD/eigenvalue rounding and the GL driver's pow/unorm rounding are
  * not reproducible here; those differences are inside the reference's own
  * float noise and are not pinned.
@@ -59,9 +73,9 @@ static void lin_table(void)
     if (g_lin_ready) return;
     for (int c = 0; c < 256; ++c) {
         float u = (float)c / 255.0f;                      /* GL unorm8 -> float */
-        if (u <= 0.04045f) g_lin[c] = u / 12.92f;          /* RGB2LAB.fs:77-78 */
+        if (u <= 0.04045f) g_lin[c] = u / 12.92f;          /* RGB2LAB.fs:27-28 */
         else {
-            float t = (u + 0.055f) / 1.055f;               /* RGB2LAB.fs:80 */
+            float t = (u + 0.055f) / 1.055f;               /* RGB2LAB.fs:30 */
             g_lin[c] = (float)pow((double)t, 2.4);
         }
     }
@@ -88,14 +102,14 @@ static double cbrt_halley(double x)
 
 static float lab_f(float r)
 {
-    /* RGB2LAB.fs:107-120: r > eps ? r^(1/3) : (kappa r + 16)/116 */
+    /* RGB2LAB.fs:50-70: r > eps ? r^(1/3) : (kappa r + 16)/116 */
     if (r > 0.008856f) return (float)cbrt_halley((double)r);
     return (903.3f * r + 16.0f) / 116.0f;
 }
 
 static uint8_t round_u8(float v)
 {
-    float r = floorf(v + 0.5f);                            /* RGB2LAB.fs:127 */
+    float r = floorf(v + 0.5f);                            /* RGB2LAB.fs:77 */
     if (!(r >= 0.0f)) r = 0.0f;
     if (r > 255.0f) r = 255.0f;
     return (uint8_t)r;
@@ -105,15 +119,15 @@ ORC_API void orc_lab_of(int r, int g, int b, uint8_t out[3])
 {
     lin_table();
     float tr = g_lin[r], tg = g_lin[g], tb = g_lin[b];
-    /* RGB2LAB.fs:69-71, :92-94 (dot products, left to right) */
+    /* RGB2LAB.fs:19-21, :42-44 (dot products, left to right) */
     float X = (tr * 0.412453f + tg * 0.357580f) + tb * 0.180423f;
     float Y = (tr * 0.212671f + tg * 0.715160f) + tb * 0.072169f;
     float Z = (tr * 0.019334f + tg * 0.119193f) + tb * 0.950227f;
-    /* RGB2LAB.fs:104-105 (D65 white) */
+    /* RGB2LAB.fs:54-55 (D65 white) */
     float fx = lab_f(X / 0.950456f);
     float fy = lab_f(Y / 1.0f);
     float fz = lab_f(Z / 1.088754f);
-    /* RGB2LAB.fs:122-127 */
+    /* RGB2LAB.fs:72-77 */
     float L = 116.0f * fy - 16.0f;
     float A = 500.0f * (fx - fy);
     float B = 200.0f * (fy - fz);
@@ -449,6 +463,17 @@ static void seed_metric(const orc_field* F, int x, int y, int z, int gate, float
     orc_nearest_psd(F->m6 + v * 6, out);
 }
 
+/* seed_metric over a given field (m6: V x 6 metric, psd: V flags, fwd:
+ * image-oriented forward flow for the gate), for constructed cases */
+ORC_API void orc_seed_metric(const float* m6, const uint8_t* psd, const float* fwd, int W, int H, int D,
+                             int x, int y, int z, int gate, float* out)
+{
+    orc_field F = {W, H, D, m6, psd, fwd};
+    seed_metric(&F, x, y, z, gate, out);
+}
+
+ORC_API int orc_angle_gate(float su, float sv, float nu, float nv) { return angle_gate(su, sv, nu, nv); }
+
 /* ------------------------------------------------------------------------ */
 /* a5: grid seeding (GPU-AS/Image.cpp:20-92)                                 */
 /* ------------------------------------------------------------------------ */
@@ -532,7 +557,7 @@ ORC_API int orc_schedule(int mode, int s, int W, int* steps, float* mult, int ca
 }
 
 /* ------------------------------------------------------------------------ */
-/* a7: one JFA pass (GPU-AS/shaders/flood.gs:53-209)                          */
+/* a7: one JFA pass (GPU-AS/shaders/flood.gs:26-182)                          */
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
@@ -543,7 +568,7 @@ typedef struct {
 
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-/* Candidate distance, flood.gs:107-121.  The reference evaluates
+/* Candidate distance, flood.gs:85-94.  The reference evaluates
  * dot((dx^2, 2dxdy, 2dxdz), m0..m2) + dot((dy^2, 2dydz, dz^2), m3..m5) and
  * clamps negatives to 0, leaving contraction and rounding to the GL driver;
  * this fixes one order of fused multiply-adds for the same quadratic form
@@ -561,13 +586,17 @@ static inline float cand_dist(const orc_seed* s, int x, int y, int z, const floa
     float t1 = fmaf(dx, s->m[0], fmaf(dy, m1d, u1));
     float t2 = fmaf(dy, s->m[3], u2);
     float q = fmaf(dx, t1, fmaf(dy, t2, u3));
-    if (signbit(q)) q = 0.0f;                                 /* flood.gs:117-120 */
+    /* flood.gs:90-93 clamps a negative value to 0 and leaves a NaN, whose
+     * distance then fails `curr_dist<min_dist` (:95): a NaN of either sign
+     * is rejected here through a +inf distance; -0 becomes +0 */
+    if (isnan(q)) return INFINITY;
+    if (!(q > 0.0f)) q = 0.0f;
     float c0 = lab[0] - s->lab[0], c1 = lab[1] - s->lab[1], c2 = lab[2] - s->lab[2];
-    float cd = fmaf(c2, c2, fmaf(c1, c1, c0 * c0));           /* flood.gs:116 */
-    return fmaf(ratio, q, cd);                                /* flood.gs:121 */
+    float cd = fmaf(c2, c2, fmaf(c1, c1, c0 * c0));           /* flood.gs:89 */
+    return fmaf(ratio, q, cd);                                /* flood.gs:94 */
 }
 
-static const int k_off9x[9] = {-1, 0, 1, -1, 0, 1, -1, 0, 1};   /* flood.gs:63-73 */
+static const int k_off9x[9] = {-1, 0, 1, -1, 0, 1, -1, 0, 1};   /* flood.gs:36-46 */
 static const int k_off9y[9] = {-1, -1, -1, 0, 0, 0, 1, 1, 1};
 
 /* tie_rule 0: lowest seed id among equal distances; 1: first in the
@@ -584,14 +613,14 @@ ORC_API void orc_jfa_pass(const int32_t* in, int32_t* out, const uint32_t* lab,
                 size_t v = ((size_t)z * H + y) * W + x;
                 uint32_t pk = lab[v];
                 float lv[3] = {(float)(pk & 255u), (float)((pk >> 8) & 255u), (float)((pk >> 16) & 255u)};
-                float best = 100000000.0f;                    /* flood.gs:90 */
+                float best = 100000000.0f;                    /* flood.gs:63 */
                 int32_t best_id = -1;
                 for (int sl = 0; sl < 3; ++sl) {
                     for (int o = 0; o < 9; ++o) {
                         int xx = clampi(x + k_off9x[o] * k, 0, W - 1);   /* CLAMP_TO_EDGE */
                         int yy = clampi(y + k_off9y[o] * k, 0, H - 1);
                         int32_t id = in[((size_t)zs[sl] * H + yy) * W + xx];
-                        if (id < 0) continue;                             /* flood.gs:104 */
+                        if (id < 0) continue;                             /* flood.gs:77 */
                         float d = cand_dist(&seeds[id], x, y, z, lv, ratio);
                         if (d < best || (tie_rule == 0 && d == best && id < best_id)) {
                             best = d;
@@ -946,8 +975,12 @@ ORC_API int orc_density_seeds(const uint8_t* img, int W, int H, int D, int K, in
     for (int j = 0; j < hN; ++j) { int lo, hi; axis_box(py, hN, H, j, &lo, &hi); for (int y = lo; y <= hi; ++y) my[y] = j; }
     for (int t = 0; t < dN; ++t) { int lo, hi; axis_box(pz, dN, D, t, &lo, &hi); for (int z = lo; z <= hi; ++z) mz[z] = t; }
 
+    /* the highest slot a float16 id can address may lie below A - 1 (A =
+     * 2146: 2145 rounds to 2144); the reference still reads all A texels
+     * (Image.cpp:524-552), and a slot nothing addresses keeps the clear
+     * colour, so the table covers max(nslot, A) slots */
     const int nslot = (int)half_round((float)(A - 1)) + 1;
-    int64_t* acc = (int64_t*)calloc((size_t)nslot, sizeof(int64_t));
+    int64_t* acc = (int64_t*)calloc((size_t)(nslot > A ? nslot : A), sizeof(int64_t));
     const float wr = 1.0f / (float)W;
     for (int z = 0; z < D; ++z)
         for (int y = 0; y < H; ++y)

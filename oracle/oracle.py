@@ -62,6 +62,9 @@ def load():
             "orc_connectivity": (i, [vp, i, i, i, i, vp]),
             "orc_half_round": (f, [f]),
             "orc_density_seeds": (i, [vp, i, i, i, i, vp, vp]),
+            "orc_seed_metric": (None, [vp, vp, vp, i, i, i, i, i, i, i, vp]),
+            "orc_angle_gate": (i, [f, f, f, f]),
+            "orc_run_seeded": (i, [vp, vp, vp, vp, vp, i, i, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -222,6 +225,44 @@ def connectivity(labels, supsz):
     if n < 0:
         raise MemoryError("orc_connectivity")
     return out, n
+
+
+def seed_metric(m6, psd, fwd, x, y, z, gate):
+    """seed_matrix's selection (GPU-AS/Image.cpp:689-872, :1351-1534) over a
+    given metric field m6 [D,H,W,6], PSD flags psd [D,H,W] and image-oriented
+    forward flow fwd [D,H,W,2] (read by the angle gate)."""
+    m6 = np.ascontiguousarray(m6, np.float32)
+    psd = np.ascontiguousarray(psd, np.uint8)
+    fwd = np.ascontiguousarray(fwd, np.float32)
+    D, H, W = psd.shape
+    out = np.zeros(6, np.float32)
+    load().orc_seed_metric(_p(m6), _p(psd), _p(fwd), W, H, D, int(x), int(y), int(z), int(gate), _p(out))
+    return out
+
+
+def angle_gate(su, sv, nu, nv):
+    return bool(load().orc_angle_gate(float(su), float(sv), float(nu), float(nv)))
+
+
+def run_seeded(rgb, fwd, bwd, xyz, s, iters, compactness=8.0, tie_rule=0, angle_gate_on=1, prepared=0,
+               want_trace=False):
+    """Plain run from explicit seed voxels xyz (A x 3) and grid step s.
+    Returns (labels [D,H,W], sums [A+1,7], trace or None)."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    fwd = np.ascontiguousarray(fwd, np.float32)
+    bwd = np.ascontiguousarray(bwd, np.float32)
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    D, H, W, _ = rgb.shape
+    A = len(xyz)
+    P = OrcParams(W, H, D, A, iters, compactness, tie_rule, angle_gate_on, 0, prepared)
+    labels = np.zeros((D, H, W), np.int32)
+    sums = np.zeros((A + 1, 7), np.int64)
+    trace = np.zeros((iters, A, 12), np.float32) if want_trace else None
+    r = load().orc_run_seeded(ctypes.byref(P), _p(rgb), _p(fwd), _p(bwd), _p(xyz), A, int(s), _p(labels), _p(sums),
+                              _p(trace) if want_trace else None)
+    if r < 0:
+        raise ValueError(f"orc_run_seeded failed: {r}")
+    return labels, sums, trace
 
 
 def half_round(x):

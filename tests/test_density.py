@@ -92,9 +92,56 @@ def test_one_changed_voxel_feeds_the_boxes_of_its_four_taps(spot):
     assert np.array_equal(dens, want)
 
 
+def brute_force_densities(img, K):
+    """Every slot's density from the painted grid volume, in numpy: the
+    gradient of derivative_t.gs:24-49 (float32 sums, float16 storage), the
+    slot of density_grid.vs:23-29 for the float16 id of the voxel's box, an
+    exact sum per slot plus the clear colour 3 (Image.cpp:484-486), over all
+    A slots (Image.cpp:524-552 reads A texels)."""
+    D, H, W, _ = img.shape
+    grid = painted_grid(W, H, D, K)
+    A = int(grid.max()) + 1
+    c = img.astype(np.int64)
+    left = np.concatenate([c[:, :, :1], c[:, :, :-1]], axis=2)
+    up = np.concatenate([c[:, :1], c[:, :-1]], axis=1)
+    prev = np.concatenate([c[:1], c[:-1]], axis=0)
+    g = np.zeros((D, H, W), np.float32)
+    for nb in (left, up, prev):
+        s = np.abs(nb - c).sum(axis=-1).astype(np.float32)
+        g = (g + np.float32(0.33) * s).astype(np.float32)
+    pg = g.astype(np.float16).astype(np.float32)
+    kr = grid.astype(np.float32).astype(np.float16).astype(np.float32)
+    wr = np.float32(1.0) / np.float32(W)
+    ty = np.floor(kr * wr).astype(np.float32)
+    tx = np.floor(kr - ty / wr).astype(np.float32)
+    slot = (ty.astype(np.int64) * W + tx.astype(np.int64)).ravel()
+    ok = (tx.ravel() >= 0) & (tx.ravel() < W) & (slot >= 0)
+    acc = np.zeros(max(A, int(slot[ok].max()) + 1), np.int64)
+    np.add.at(acc, slot[ok], (pg.ravel()[ok] * 4096.0).astype(np.int64))
+    return (3.0 + acc[:A] / 4096.0).astype(np.float32)
+
+
+def test_densities_cover_every_slot_when_float16_ids_stop_short():
+    """320x240x20, K=2149: A = 2146 seeds, but the last id 2145 is odd and
+    rounds to 2144 in float16, so no voxel addresses slot 2145.  The slot
+    table must still cover all A slots (the last one at the clear density
+    3), and every density equals the brute-force painting."""
+    W, H, D, K = 320, 240, 20, 2149
+    A = len(orc.grid_seeds(W, H, D, K)[0])
+    assert A == 2146 and int(np.float16(A - 1)) + 1 < A
+    rgb, _, _, _ = asv.synth_video(37, W, H, D, 10, 2.0, 4.0)
+    img = orc.blur_flip(rgb)
+    xyz, dens = orc.density_seeds(img, K, prepared=1)
+    assert len(xyz) == A
+    assert dens[A - 1] == 3.0
+    assert np.array_equal(dens, brute_force_densities(img, K))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("seed,W,H,D,K,prepared", [
     (31, 48, 40, 12, 60, 0), (32, 160, 120, 30, 500, 0), (33, 96, 64, 20, 300, 1), (34, 320, 240, 20, 3000, 0),
+    # the float16 rounding of the last id lies below it (A = 2146, 2145 -> 2144)
+    (37, 320, 240, 20, 2149, 0),
 ])
 def test_device_density_seeds_match_oracle(have_gpu, seed, W, H, D, K, prepared):
     rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 8, 2.0, 4.0)

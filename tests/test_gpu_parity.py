@@ -158,6 +158,45 @@ def test_small_runs_bit_exact(have_gpu, seed, shape, K, iters, kw):
         "frames_prepared": kw.get("frames_prepared", 0)})
 
 
+def test_nan_flow_bit_exact(have_gpu):
+    """x86 default NaNs (0xFFC00000) in the flow reach the least squares of
+    the seeds and their rings (the pseudo-inverse then returns zero metrics)
+    and the angle gate (a NaN cosine fails it): labels, sums and every seed
+    record bit-exact, NaNs compared as NaNs."""
+    W, H, D, K, iters = 48, 40, 10, 60, 3
+    rgb, fwd, bwd, _ = asv.synth_video(17, W, H, D, 5, 2.0, 4.0)
+    fwd, bwd = fwd.copy(), bwd.copy()
+    nan = np.array([0xFFC00000], np.uint32).view(np.float32)[0]
+    fwd[2:8, 8:30:3, 10:40:4] = nan
+    bwd[3:9, 12:34:5, 6:30:3] = nan
+    got = _clip_run(rgb, fwd, bwd, K, iters)
+    want_labels, want_sums, want_trace, A = orc.run(rgb, fwd, bwd, K, iters, want_trace=True)
+    assert got["A"] == A
+    tr = got["trace"]
+    assert np.array_equal(np.isnan(tr), np.isnan(want_trace))
+    assert np.array_equal(np.nan_to_num(tr, nan=0.0), np.nan_to_num(want_trace, nan=0.0))
+    assert int((got["labels"] != want_labels).sum()) == 0
+    assert np.array_equal(got["sums"][:, :7], want_sums)
+
+
+@pytest.mark.parametrize("seed,shape,K,iters", [(18, (10, 40, 48), 60, 3), (19, (24, 64, 80), 60, 2)])
+def test_exact_clamp_path_bit_exact(have_gpu, seed, shape, K, iters):
+    """The marching pass's exact per-tap path, which a run takes when a seed
+    metric could make a quadratic form NaN (non-finite or beyond 2^60; flows
+    cannot produce one, so the flag is forced here): the same labels, sums
+    and records as the oracle."""
+    D, H, W = shape
+    rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 6, 2.0, 4.0)
+    p = asv.default_params(W, H, D, K, lloyd_iters=iters)
+    with asv.Clip(p) as c:
+        c.enable_trace()
+        c.force_exact_clamp(True)
+        c.upload(rgb, fwd, bwd)
+        c.run()
+        got = dict(labels=c.labels(), sums=c.sums(), A=c.A, trace=c.trace())
+    _assert_run_equal(got, rgb, fwd, bwd, K, iters)
+
+
 def test_many_seeds_hash_table_path(have_gpu):
     """More seeds than the direct-mapped LDS id table holds (A > 16384):
     the tiles use the hashed candidate table."""
