@@ -47,6 +47,7 @@ SIGNATURES = {
     "asv_download_sums": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_enable_trace": (ctypes.c_int, [_vp, _i32]),
     "asv_force_exact_clamp": (ctypes.c_int, [_vp, _i32]),
+    "asv_debug_fail_alloc": (ctypes.c_int, [_i64]),
     "asv_download_trace": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_lab": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_metric": (ctypes.c_int, [_vp, _vp, _vp]),
@@ -130,7 +131,8 @@ class ServeParams(ctypes.Structure):
 class ServeRequest(ctypes.Structure):
     _fields_ = [("stream_id", _i32), ("width", _i32), ("height", _i32), ("frames", _i32), ("density", _f32),
                 ("n_ellipses", _i32), ("video_seed", ctypes.c_uint64), ("vmax", _f32), ("noise_sigma", _f32),
-                ("arrival_ms", ctypes.c_double), ("rgb", _vp), ("fwd", _vp), ("bwd", _vp)]
+                ("arrival_ms", ctypes.c_double), ("rgb", _vp), ("fwd", _vp), ("bwd", _vp), ("cancel", _vp),
+                ("cancel_after_windows", _i32)]
 
 
 class ServeWindow(ctypes.Structure):

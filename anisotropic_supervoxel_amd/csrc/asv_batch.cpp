@@ -288,7 +288,7 @@ int asv_batch_create(const asv_params* params, int32_t n, asv_batch** out)
     hipError_t e = hipStreamCreate(&b->st);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
-    if (e == hipSuccess && b->n_desc) e = hipMalloc((void**)&b->dev_desc, sizeof(JfaClip) * b->n_desc);
+    if (e == hipSuccess && b->n_desc) e = asv::dev_malloc((void**)&b->dev_desc, sizeof(JfaClip) * b->n_desc);
     if (e != hipSuccess) {
         delete b;
         return bfail(ASV_ERR_HIP, "batch set-up", hipGetErrorString(e));

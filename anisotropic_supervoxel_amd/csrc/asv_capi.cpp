@@ -7,6 +7,7 @@
 #include "asv_ctx.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,10 @@ using namespace asv;
 namespace {
 
 thread_local std::string g_err;
+
+// asv_debug_fail_alloc's threshold: allocations of at least this many bytes
+// fail (0: none).
+std::atomic<int64_t> g_fail_alloc_from{0};
 
 int fail(int code, const char* fmt, const char* detail = nullptr)
 {
@@ -120,13 +125,27 @@ void lin_table(float* lut)
     }
 }
 
+}  // namespace
+
+hipError_t asv::dev_malloc(void** p, size_t bytes)
+{
+    const int64_t from = g_fail_alloc_from.load(std::memory_order_relaxed);
+    if (from > 0 && (int64_t)bytes >= from) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
+namespace {
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count, int64_t* total)
 {
     size_t bytes = count * sizeof(T);
     if (bytes == 0) bytes = 16;
     *total += (int64_t)bytes;
-    return hipMalloc((void**)p, bytes);
+    return dev_malloc((void**)p, bytes);
 }
 
 bool valid_params(const asv_params* p)
@@ -361,8 +380,8 @@ int density_reseed(asv_ctx* c, hipStream_t st)
     int* d_maps = nullptr;
     unsigned long long* d_acc = nullptr;
     std::vector<unsigned long long> acc((size_t)std::max(nslot, A), 0ull);
-    hipError_t e = hipMalloc((void**)&d_maps, sizeof(int) * maps.size());
-    if (e == hipSuccess) e = hipMalloc((void**)&d_acc, sizeof(unsigned long long) * (size_t)nslot);
+    hipError_t e = asv::dev_malloc((void**)&d_maps, sizeof(int) * maps.size());
+    if (e == hipSuccess) e = asv::dev_malloc((void**)&d_acc, sizeof(unsigned long long) * (size_t)nslot);
     if (e == hipSuccess) e = hipMemcpyAsync(d_maps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = launch_zero_u64(st, reinterpret_cast<uint64_t*>(d_acc), nslot);
     if (e == hipSuccess)
@@ -767,7 +786,7 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
         if (c->map_table) (void)hipFree(c->map_table);
         c->map_table = nullptr;
         c->map_cap = 0;
-        HIP_TRY(hipMalloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
+        HIP_TRY(asv::dev_malloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
         c->map_cap = n_table;
     }
     HIP_TRY(hipMemcpyAsync(c->map_table, table, sizeof(int32_t) * (size_t)n_table, hipMemcpyDefault, st));
@@ -776,7 +795,7 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
     HIP_TRY(launch_map_labels(st, c->labels[c->cur], scratch, n, c->map_table, n_table));
     HIP_TRY(hipMemcpyAsync(labels, scratch, (size_t)n * 4, hipMemcpyDefault, st));
     if (checksum) {
-        if (!c->ck_word) HIP_TRY(hipMalloc((void**)&c->ck_word, sizeof(unsigned long long)));
+        if (!c->ck_word) HIP_TRY(asv::dev_malloc((void**)&c->ck_word, sizeof(unsigned long long)));
         unsigned long long* ck = c->ck_word;
         unsigned long long* host = c->sums_host + (size_t)(c->seed_cap + 1) * kSumWords;
         HIP_TRY(launch_zero_u64(st, reinterpret_cast<uint64_t*>(ck), 1));
@@ -859,7 +878,7 @@ int asv_postprocess_device(asv_ctx* c, const int64_t* sums, int32_t* dead_out, i
         if (c->post_scratch) (void)hipFree(c->post_scratch);
         c->post_scratch = nullptr;
         c->post_cap = 0;
-        HIP_TRY(hipMalloc((void**)&c->post_scratch, sizeof(long long) * (size_t)words));
+        HIP_TRY(asv::dev_malloc((void**)&c->post_scratch, sizeof(long long) * (size_t)words));
         c->post_cap = words;
     }
     const int n_table = (int)pl.table.size();
@@ -867,7 +886,7 @@ int asv_postprocess_device(asv_ctx* c, const int64_t* sums, int32_t* dead_out, i
         if (c->map_table) (void)hipFree(c->map_table);
         c->map_table = nullptr;
         c->map_cap = 0;
-        HIP_TRY(hipMalloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
+        HIP_TRY(asv::dev_malloc((void**)&c->map_table, sizeof(int32_t) * (size_t)n_table));
         c->map_cap = n_table;
     }
     if (n_table > 0)
@@ -897,10 +916,10 @@ int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, in
     long long* scratch = nullptr;
     int32_t* table = nullptr;
     const int n_table = (int)pl.table.size();
-    HIP_TRY(hipMalloc((void**)&scratch, sizeof(long long) * (size_t)words));
+    HIP_TRY(asv::dev_malloc((void**)&scratch, sizeof(long long) * (size_t)words));
     hipError_t e = hipSuccess;
     if (n_table > 0) {
-        e = hipMalloc((void**)&table, sizeof(int32_t) * (size_t)n_table);
+        e = asv::dev_malloc((void**)&table, sizeof(int32_t) * (size_t)n_table);
         if (e == hipSuccess)
             e = hipMemcpyAsync(table, pl.table.data(), sizeof(int32_t) * (size_t)n_table, hipMemcpyDefault, st);
     }
@@ -988,6 +1007,13 @@ int asv_download_sums(asv_ctx* c, int64_t* sums, void* stream)
     return ASV_OK;
 }
 
+int asv_debug_fail_alloc(int64_t min_bytes)
+{
+    if (min_bytes < 0) return fail(ASV_ERR_ARG, "negative allocation threshold");
+    g_fail_alloc_from.store(min_bytes, std::memory_order_relaxed);
+    return ASV_OK;
+}
+
 int asv_force_exact_clamp(asv_ctx* c, int32_t on)
 {
     if (!c) return fail(ASV_ERR_ARG, "null context");
@@ -1000,7 +1026,7 @@ int asv_enable_trace(asv_ctx* c, int32_t on)
     if (!c) return fail(ASV_ERR_ARG, "null context");
     if (on && !c->trace) {
         size_t n = (size_t)c->p.lloyd_iters * c->d.A * 12;
-        hipError_t e = hipMalloc((void**)&c->trace, n * sizeof(float));
+        hipError_t e = asv::dev_malloc((void**)&c->trace, n * sizeof(float));
         if (e != hipSuccess) { c->trace = nullptr; return fail(ASV_ERR_NOMEM, "trace allocation failed"); }
         c->bytes += (int64_t)(n * sizeof(float));
     }
@@ -1033,7 +1059,7 @@ int asv_download_metric(asv_ctx* c, float* m8, void* stream)
     if (!c->prepared) return fail(ASV_ERR_STATE, "asv_prepare has not run");
     hipStream_t st = S(stream);
     float4* field = nullptr;
-    HIP_TRY(hipMalloc(&field, (size_t)c->V() * 32));
+    HIP_TRY(asv::dev_malloc((void**)&field, (size_t)c->V() * 32));
     hipError_t e = launch_metric_field(st, c->d, c->fwd, c->bwd, field);
     if (e == hipSuccess) e = hipMemcpyAsync(m8, field, (size_t)c->V() * 32, hipMemcpyDefault, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

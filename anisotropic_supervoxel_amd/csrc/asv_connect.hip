@@ -259,11 +259,11 @@ hipError_t connect_labels(hipStream_t st, const int32_t* in, int32_t* out, int W
     const int64_t nchunk = (V + kCcChunk - 1) / kCcChunk;
     int *parent = nullptr, *rank_at = nullptr, *roots = nullptr, *counts = nullptr, *offsets = nullptr;
     int *size = nullptr, *adj = nullptr, *fin = nullptr;
-    hipError_t e = hipMalloc((void**)&parent, sizeof(int) * (size_t)V);
-    if (e == hipSuccess) e = hipMalloc((void**)&rank_at, sizeof(int) * (size_t)V);
-    if (e == hipSuccess) e = hipMalloc((void**)&roots, sizeof(int) * (size_t)V);
-    if (e == hipSuccess) e = hipMalloc((void**)&counts, sizeof(int) * (size_t)nchunk);
-    if (e == hipSuccess) e = hipMalloc((void**)&offsets, sizeof(int) * (size_t)(nchunk + 1));
+    hipError_t e = asv::dev_malloc((void**)&parent, sizeof(int) * (size_t)V);
+    if (e == hipSuccess) e = asv::dev_malloc((void**)&rank_at, sizeof(int) * (size_t)V);
+    if (e == hipSuccess) e = asv::dev_malloc((void**)&roots, sizeof(int) * (size_t)V);
+    if (e == hipSuccess) e = asv::dev_malloc((void**)&counts, sizeof(int) * (size_t)nchunk);
+    if (e == hipSuccess) e = asv::dev_malloc((void**)&offsets, sizeof(int) * (size_t)(nchunk + 1));
     int n = 0;
     std::vector<int> h_size, h_adj, h_fin;
     if (e == hipSuccess) {
@@ -278,9 +278,9 @@ hipError_t connect_labels(hipStream_t st, const int32_t* in, int32_t* out, int W
     if (e == hipSuccess) e = hipMemcpyAsync(&n, offsets + nchunk, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e == hipSuccess && n > 0) {
-        e = hipMalloc((void**)&size, sizeof(int) * (size_t)n);
-        if (e == hipSuccess) e = hipMalloc((void**)&adj, sizeof(int) * (size_t)n);
-        if (e == hipSuccess) e = hipMalloc((void**)&fin, sizeof(int) * (size_t)n);
+        e = asv::dev_malloc((void**)&size, sizeof(int) * (size_t)n);
+        if (e == hipSuccess) e = asv::dev_malloc((void**)&adj, sizeof(int) * (size_t)n);
+        if (e == hipSuccess) e = asv::dev_malloc((void**)&fin, sizeof(int) * (size_t)n);
         if (e == hipSuccess) e = hipMemsetAsync(size, 0, sizeof(int) * (size_t)n, st);
         if (e == hipSuccess) {
             const int64_t th = (V + 15) / 16;

@@ -9,6 +9,11 @@
 
 namespace asv {
 
+// Every device allocation of the library goes through here: hipMalloc,
+// unless a test has asked allocations of at least some size to fail
+// (asv_debug_fail_alloc), in which case it returns hipErrorOutOfMemory.
+hipError_t dev_malloc(void** p, size_t bytes);
+
 // One seed as the flood reads it: 48 bytes, three 16-byte loads.
 //   a = (x, y, z, m0)   voxel coordinates as exact floats
 //   b = (m1, m2, m3, m4)

@@ -2005,12 +2005,11 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
 // (31 or 63) overrides the choice for every step.
 int march_mc(int k)
 {
-    static int forced = -1;
-    if (forced < 0) {
+    static const int forced = [] {          // read once, thread-safe initialisation
         const char* e = getenv("ASV_MARCH_MC");
         const int v = e ? atoi(e) : 0;
-        forced = (v == 31 || v == 63) ? v : 0;
-    }
+        return (v == 31 || v == 63) ? v : 0;
+    }();
     if (forced) return forced;
     // measured at 640x480x200 (31 vs 63): K = 1 313 vs 354 us, K = 2 332 vs
     // 371, K = 3 480 vs 513, K = 5 430 vs 425 (two runs each, within 0.3 %)
@@ -2023,12 +2022,11 @@ int march_mc(int k)
 // (1 or 2) overrides the choice for every step.
 int march_s(int k)
 {
-    static int forced = -1;
-    if (forced < 0) {
+    static const int forced = [] {
         const char* e = getenv("ASV_MARCH_S");
         const int v = e ? atoi(e) : 0;
-        forced = (v == 1 || v == 2) ? v : 0;
-    }
+        return (v == 1 || v == 2) ? v : 0;
+    }();
     if (forced) return forced;
     // measured at 640x480x200: K = 1 362 vs 335 us, K = 2 420 vs 383,
     // K = 3 622 vs 566, K = 5 573 vs 511 (one vs two slices per step)

@@ -33,6 +33,15 @@ double ms_since(clock_t_::time_point t0)
     return std::chrono::duration<double, std::milli>(clock_t_::now() - t0).count();
 }
 
+// A request the loop can run at all: sizes, a finite positive density, and
+// either all three host arrays or none.
+bool well_formed(const asv_serve_request* r)
+{
+    const int have = (r->rgb != nullptr) + (r->fwd != nullptr) + (r->bwd != nullptr);
+    return r->width >= 2 && r->height >= 2 && r->frames >= 1 && r->density > 0.0f &&
+           r->density < 1e6f && r->n_ellipses >= 0 && (have == 0 || have == 3);
+}
+
 asv_stream_params stream_params_of(const asv_serve_params* sp, const asv_serve_request* r)
 {
     asv_stream_params p;
@@ -67,7 +76,7 @@ void asv_serve_default_params(asv_serve_params* p)
 
 int64_t asv_serve_footprint(const asv_serve_params* sp, const asv_serve_request* r)
 {
-    if (!sp || !r || r->width < 2 || r->height < 2 || r->frames < 1) return -1;
+    if (!sp || !r || !well_formed(r)) return -1;
     asv_stream_params p = stream_params_of(sp, r);
     int32_t iv[2 * 1024];
     int nb = asv_stream_plan(&p, iv, 1024);
@@ -165,6 +174,13 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         uint64_t vox_iters = 0;
         double ready = ms_since(t0);
         for (int b = 0; b < nb && !rc; ++b) {
+            // a cancelled request stops between windows; its context stays
+            // consistent (the last window completed) and goes back to the
+            // idle pool like a finished stream's
+            if ((r.cancel && *r.cancel) || (r.cancel_after_windows > 0 && b >= r.cancel_after_windows)) {
+                rc = ASV_ERR_CANCELLED;
+                break;
+            }
             const int f0 = iv[2 * (size_t)b], f1 = iv[2 * (size_t)b + 1];
             const size_t nf = (size_t)(f1 - f0);
             std::vector<uint8_t> rgb_buf;
@@ -211,7 +227,9 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             results[ri].labels_hash = ck;
             results[ri].voxel_iters = (int64_t)vox_iters;
         }
-        asv_ctx* left = rc ? nullptr : asv::stream_take_ctx(s);
+        // after an error the context may hold a partial window: it is freed
+        // with the stream instead of being offered to later streams
+        asv_ctx* left = (rc == ASV_OK || rc == ASV_ERR_CANCELLED) ? asv::stream_take_ctx(s) : nullptr;
         asv_stream_destroy(s);
         if (left) {
             asv_ctx* evict = nullptr;
@@ -276,8 +294,8 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
                     continue;
                 }
                 const int64_t fp = asv_serve_footprint(sp, &r);
-                if (fp < 0 || fp > sp->memory_budget) {          // can never be admitted
-                    results[ri].status = ASV_ERR_ADMIT;
+                if (fp < 0 || fp > sp->memory_budget) {          // malformed, or can never be admitted
+                    results[ri].status = fp < 0 ? ASV_ERR_ARG : ASV_ERR_ADMIT;
                     results[ri].footprint = fp;
                     it = pending.erase(it);
                     continue;
@@ -321,8 +339,8 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     for (const Idle& e : idle) asv_destroy(e.c);
     if (n_windows) *n_windows = nwin;
     if (wall_ms) *wall_ms = ms_since(t0);
-    for (int i = 0; i < n; ++i)
-        if (results[i].status != ASV_OK) return results[i].status == ASV_ERR_ADMIT ? ASV_OK : results[i].status;
+    // each request's outcome is in its result; one request's failure ends
+    // only that request
     return ASV_OK;
 }
 

@@ -43,15 +43,24 @@ def make_requests(n, rate_per_s=40.0, seed=0, resolutions=None, frames=(60, 300)
     return out
 
 
-_FIELDS = [f[0] for f in ServeRequest._fields_ if f[0] not in ("rgb", "fwd", "bwd")]
+_FIELDS = [f[0] for f in ServeRequest._fields_ if f[0] not in ("rgb", "fwd", "bwd", "cancel",
+                                                                  "cancel_after_windows")]
 
 
 def _request(r):
+    """A ServeRequest from a request dict; optional keys: frames_data (rgb,
+    fwd, bwd arrays), cancel (a one-element int32 array the caller may set
+    while the loop runs), cancel_after_windows."""
     q = ServeRequest(**{k: r[k] for k in _FIELDS})
     frames = r.get("frames_data")
     if frames is not None:
         rgb, fwd, bwd = frames
         q.rgb, q.fwd, q.bwd = rgb.ctypes.data, fwd.ctypes.data, bwd.ctypes.data
+    flag = r.get("cancel")
+    if flag is not None:
+        assert flag.dtype == np.int32 and flag.size >= 1
+        q.cancel = flag.ctypes.data
+    q.cancel_after_windows = int(r.get("cancel_after_windows", 0))
     return q
 
 

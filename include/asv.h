@@ -28,7 +28,8 @@ enum {
     ASV_ERR_HIP = -2,      /* a HIP runtime call failed */
     ASV_ERR_STATE = -3,    /* call out of order (e.g. run before upload) */
     ASV_ERR_NOMEM = -4,    /* device allocation failed */
-    ASV_ERR_ADMIT = -5     /* serving: footprint exceeds the memory budget */
+    ASV_ERR_ADMIT = -5,    /* serving: footprint exceeds the memory budget */
+    ASV_ERR_CANCELLED = -6 /* serving: the request was cancelled between windows */
 };
 
 enum { ASV_TIE_LOWEST_ID = 0, ASV_TIE_SCAN_ORDER = 1 };
@@ -222,6 +223,11 @@ int asv_host_free(void* p);
  * every later run of the context (for tests and tools); no counterpart in
  * the reference. */
 int asv_force_exact_clamp(asv_ctx* ctx, int32_t on);
+/* Fault injection for tests of failure handling: from now on every device
+ * allocation of at least min_bytes fails as hipErrorOutOfMemory would
+ * (0 restores normal allocation).  Process-wide; no counterpart in the
+ * reference. */
+int asv_debug_fail_alloc(int64_t min_bytes);
 /* Per-iteration seed records, when enabled before the run:
  * lloyd_iters x A x 12 float (x, y, z, L, a, b, m0..m5). */
 int asv_enable_trace(asv_ctx* ctx, int32_t on);
@@ -384,6 +390,12 @@ typedef struct asv_serve_request {
     const uint8_t* rgb;
     const float* fwd;
     const float* bwd;
+    /* cancellation, checked before each window: the request ends with
+     * ASV_ERR_CANCELLED once *cancel is non-zero (a flag the caller may set
+     * from any thread; NULL: none) or once cancel_after_windows windows are
+     * done (0: never) */
+    const volatile int32_t* cancel;
+    int32_t cancel_after_windows;
 } asv_serve_request;
 
 typedef struct asv_serve_window {
@@ -393,7 +405,9 @@ typedef struct asv_serve_window {
 
 typedef struct asv_serve_result {
     int32_t stream_id;
-    int32_t status;             /* ASV_OK, ASV_ERR_ADMIT (never fits the budget) or an error */
+    int32_t status;             /* ASV_OK, ASV_ERR_ADMIT (never fits the budget), ASV_ERR_ARG
+                                   (malformed request), ASV_ERR_CANCELLED or the error that
+                                   ended the stream; the other streams are unaffected */
     uint64_t labels_hash;       /* asv_stream_checksum of the stitched global labels */
     int64_t footprint;          /* device bytes admitted for the stream */
     int64_t voxel_iters;
@@ -402,6 +416,11 @@ typedef struct asv_serve_result {
 
 void asv_serve_default_params(asv_serve_params* p);
 int64_t asv_serve_footprint(const asv_serve_params* p, const asv_serve_request* r);
+/* Runs the requests to completion.  Returns ASV_ERR_ARG for unusable
+ * arguments, else ASV_OK: every request's own outcome is in its result (a
+ * malformed, refused, cancelled or failing request ends alone, the others
+ * run on), and the windows (up to window_cap) are recorded in the order
+ * they finish. */
 int asv_serve_run(const asv_serve_params* p, const asv_serve_request* reqs, int32_t n, asv_serve_result* results,
                   asv_serve_window* windows, int32_t window_cap, int32_t* n_windows, double* wall_ms);
 

@@ -59,6 +59,43 @@ def test_served_mix_equals_alone_and_oracle(have_gpu):
 
 
 @pytest.mark.gpu
+def test_failures_end_only_their_own_request(have_gpu):
+    """A malformed request (width 1), a request cancelled after two windows,
+    one cancelled by its flag before it starts, and one whose device
+    allocation fails (every allocation of 1 MiB or more fails as
+    hipErrorOutOfMemory, which only the 160 x 120 stream's flow buffers
+    reach): each ends with its own status while every other stream completes
+    with the checksum it has alone."""
+    lib = asv._lib.load()
+    reqs = _small_requests()
+    alone = {}
+    for r in reqs:
+        res, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), [r])
+        assert res[0]["status"] == 0
+        alone[r["stream_id"]] = res[0]["labels_hash"]
+    flag = np.zeros(1, np.int32)
+    mix = [dict(r) for r in reqs]
+    mix.append(dict(reqs[0], stream_id=90, width=1))                        # malformed
+    mix.append(dict(reqs[2], stream_id=91, cancel_after_windows=2))         # 47 frames: 8 windows
+    flag[0] = 1
+    mix.append(dict(reqs[1], stream_id=92, cancel=flag))                    # cancelled before it starts
+    mix.append(dict(reqs[3], stream_id=93, width=160, height=120))          # allocation fails
+    try:
+        asv._lib.check(lib.asv_debug_fail_alloc(1 << 20))
+        res, windows, _ = serve.run(serve.default_params(max_active=3, memory_budget=8 << 30), mix)
+    finally:
+        lib.asv_debug_fail_alloc(0)
+    by_id = {r["stream_id"]: r for r in res}
+    assert by_id[90]["status"] == -1
+    assert by_id[91]["status"] == -6 and sum(w["stream_id"] == 91 for w in windows) == 2
+    assert by_id[92]["status"] == -6 and not any(w["stream_id"] == 92 for w in windows)
+    assert by_id[93]["status"] == -4
+    for r in reqs:
+        assert by_id[r["stream_id"]]["status"] == 0
+        assert by_id[r["stream_id"]]["labels_hash"] == alone[r["stream_id"]], r
+
+
+@pytest.mark.gpu
 def test_admission_limits_concurrency(have_gpu):
     reqs = _small_requests()
     p0 = serve.default_params()
