@@ -80,3 +80,61 @@ def test_batch_matches_oracle(have_gpu):
             assert np.array_equal(b.sums(i)[:, :7], sums[:, :7])
     finally:
         b.close()
+
+
+def _stratified(clips):
+    """Indices of the config-4 clips checked against the oracle: the
+    smallest clip of each frame size, the smallest and largest K, the
+    longest clip and the largest clip."""
+    pick = set()
+    for W, H in batch.SIZES:
+        same = [i for i, c in enumerate(clips) if (c["W"], c["H"]) == (W, H)]
+        if same:
+            pick.add(min(same, key=lambda i: clips[i]["D"]))
+    vol = [c["W"] * c["H"] * c["D"] for c in clips]
+    pick.add(min(range(len(clips)), key=lambda i: clips[i]["K"]))
+    pick.add(max(range(len(clips)), key=lambda i: clips[i]["K"]))
+    pick.add(max(range(len(clips)), key=lambda i: clips[i]["D"]))
+    pick.add(max(range(len(clips)), key=lambda i: vol[i]))
+    return sorted(pick)
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_config4_64_clips(have_gpu):
+    """The batched benchmark workload itself: 64 clips (video seeds 100-163,
+    160x120 to 640x480, 20-120 frames, K 200-4000, 10 iterations, 644 M
+    voxels) in one batch, labels and sums of every clip bit-identical to its
+    own single-clip run, over two batched runs (the second replays the
+    captured graphs), and a stratified subset of eight clips equal to the
+    oracle."""
+    clips = batch.make_clips()
+    params = [batch.clip_params(c) for c in clips]
+    single = []
+    b = batch.Batch(params)
+    try:
+        for i, (c, p) in enumerate(zip(clips, params)):
+            rgb, fwd, bwd, _ = batch.clip_video(c)
+            with asv.Clip(p) as cl:
+                cl.upload(rgb, fwd, bwd)
+                cl.run()
+                single.append((cl.labels(), cl.sums()))
+            b.upload(i, rgb, fwd, bwd)
+            print(f"config 4: clip {i} ({c['W']}x{c['H']}x{c['D']}, K={c['K']}) run alone", flush=True)
+        for rep in range(2):
+            b.run()
+            for i in range(len(clips)):
+                assert np.array_equal(b.labels(i), single[i][0]), (rep, i)
+                assert np.array_equal(b.sums(i), single[i][1]), (rep, i)
+    finally:
+        b.close()
+    picks = _stratified(clips)
+    assert len(picks) >= 6
+    for i in picks:
+        c = clips[i]
+        rgb, fwd, bwd, _ = batch.clip_video(c)
+        labels, sums, _, _ = orc.run(rgb, fwd, bwd, c["K"], c["iters"])
+        print(f"config 4: clip {i} checked against the oracle", flush=True)
+        assert np.array_equal(single[i][0], labels), c
+        assert np.array_equal(single[i][1][:, :7], sums), c

@@ -180,3 +180,30 @@ def test_stream_checksum_matches_stitched_labels(have_gpu):
     want, _ = ostream.run_video(rgb, fwd, bwd, K, 10, 5, 4)
     assert np.array_equal(out, want)
     assert ck.value == serve.labels_checksum(want)
+
+
+@pytest.mark.gpu
+@pytest.mark.slow
+@pytest.mark.timeout(900)
+def test_config5_resolutions_alone_mix_oracle(have_gpu):
+    """Served streams at the workload's own resolutions: a 640x480 and a
+    1280x720 stream at density 4 per 1000 pixels per frame (1280x720
+    windows hold up to ~37 k seeds), a 320x240 stream at density 1, 20-25
+    frames each, windows of 10 with stride 5, 5 iterations.  Each stream's
+    stitched labels are the same alone and in the mix, and equal to the
+    oracle's block-by-block restatement."""
+    spec = [(640, 480, 20, 4.0), (1280, 720, 20, 4.0), (320, 240, 25, 1.0)]
+    reqs = [dict(stream_id=i, width=W, height=H, frames=D, density=d, n_ellipses=16, video_seed=700 + i,
+                 vmax=2.0, noise_sigma=4.0, arrival_ms=0.0) for i, (W, H, D, d) in enumerate(spec)]
+    mix, windows, _ = serve.run(serve.default_params(max_active=3, honor_arrivals=0), reqs)
+    assert all(r["status"] == 0 for r in mix)
+    assert max(w["seeds"] for w in windows if w["stream_id"] == 1) > 16384
+    by_id = {r["stream_id"]: r for r in mix}
+    for r in reqs:
+        alone, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), [r])
+        assert alone[0]["labels_hash"] == by_id[r["stream_id"]]["labels_hash"], r
+        rgb, fwd, bwd, _ = asv.synth_video(r["video_seed"], r["width"], r["height"], r["frames"], r["n_ellipses"],
+                                           r["vmax"], r["noise_sigma"])
+        seeds100 = int(r["density"] * r["width"] * r["height"] / 10.0)
+        want, _ = ostream.run_video(rgb, fwd, bwd, 1, block_size=10, overlap=5, iters=5, seeds_per_100=seeds100)
+        assert serve.labels_checksum(want) == by_id[r["stream_id"]]["labels_hash"], r
