@@ -66,6 +66,7 @@ SIGNATURES = {
     "asv_stream_checksum": (ctypes.c_int, [_vp, _vp]),
     "asv_postprocess_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "asv_postprocess_labels": (ctypes.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "asv_postprocess_labels_noalloc": (ctypes.c_int, [_vp, _i32, _vp, _vp, _i64, _vp, _vp, _vp]),
     "asv_host_alloc": (ctypes.c_int, [_i64, ctypes.POINTER(_vp)]),
     "asv_host_free": (ctypes.c_int, [_vp]),
     "asv_reseed": (ctypes.c_int, [_vp, _vp, _i32, _i32, _i32, _vp]),

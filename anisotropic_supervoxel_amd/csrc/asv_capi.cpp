@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -928,6 +929,77 @@ int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, in
     (void)hipFree(scratch);
     if (table) (void)hipFree(table);
     if (e != hipSuccess) return fail(ASV_ERR_HIP, "post-process: %s", hipGetErrorString(e));
+    return ASV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch of asv_postprocess_labels_noalloc: one copy of this array per
+// device, held until the process ends.  Words [0, kNaChunkWords) take the
+// scan's chunk maxima (one 64-bit word per 4096 voxels), the rest the label
+// table (one int32 per seed).  Host code uses only the device address that
+// hipGetSymbolAddress returns for the current device, never the array's
+// name (a host-side stand-in).
+constexpr int64_t kNaChunkWords = 131072;         // volumes up to 131072 x 4096 = 2^29 voxels
+constexpr int64_t kNaTableInts = 65536;           // labels 0 .. 65535
+constexpr int64_t kNaWords = kNaChunkWords + kNaTableInts / 2;
+static __device__ __align__(256) unsigned long long scratch[kNaWords];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};
+
+// This device's address of `scratch`, looked up on the first call on the
+// device (with the device current) and kept.
+hipError_t noalloc_scratch(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNaMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_na_ptr[dev] = p;
+    }
+    *out = g_na_ptr[dev];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int asv_postprocess_labels_noalloc(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
+                                   int32_t* dead_out, int32_t* final_label_num, void* stream)
+{
+    if (!sums || !in || !out || A < 1 || V < 1 || in == out) return fail(ASV_ERR_ARG, "bad post-process arguments");
+    if (post_scratch_words(V) > kNaChunkWords)
+        return fail(ASV_ERR_ARG, "volume exceeds the no-allocation scratch (at most 2^29 voxels)");
+    if (A > kNaTableInts) return fail(ASV_ERR_ARG, "more seeds than the no-allocation scratch holds (65536)");
+    PostPlan pl = post_plan(sums, A);
+    if (dead_out) *dead_out = pl.dead;
+    if (final_label_num) *final_label_num = pl.valid;
+    hipStream_t st = S(stream);
+    if (pl.mode < 0) {
+        HIP_TRY(hipMemcpyAsync(out, in, sizeof(int32_t) * (size_t)V, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return ASV_OK;
+    }
+    void* base = nullptr;
+    HIP_TRY(noalloc_scratch(&base));
+    long long* chunk = static_cast<long long*>(base);
+    int32_t* table = reinterpret_cast<int32_t*>(static_cast<unsigned long long*>(base) + kNaChunkWords);
+    const int n_table = (int)pl.table.size();
+    if (n_table > 0)
+        HIP_TRY(hipMemcpyAsync(table, pl.table.data(), sizeof(int32_t) * (size_t)n_table, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_postprocess(st, in, out, V, pl.mode, table, n_table, chunk));
+    HIP_TRY(hipStreamSynchronize(st));          // the table's host copy is local
     return ASV_OK;
 }
 

@@ -182,6 +182,16 @@ int asv_postprocess_device(asv_ctx* ctx, const int64_t* sums, int32_t* dead_out,
  * GL raster order, for a label volume produced elsewhere.  Synchronises. */
 int asv_postprocess_labels(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
                            int32_t* dead_out, int32_t* final_label_num, void* stream);
+/* No counterpart in the reference: asv_postprocess_labels for callers that
+ * cannot allocate device memory (a serving loop that must not call
+ * hipMalloc, harnesses that forbid it).  Same arguments, kernels and
+ * results; its scratch is a fixed 1.25 MiB array per device, held until the
+ * process ends, so V is at most 2^29 voxels and A at most 65536 (larger
+ * sizes return ASV_ERR_ARG).  Calls on one device share that array: calls
+ * that overlap on one device (other threads, other streams) race.
+ * Synchronises stream. */
+int asv_postprocess_labels_noalloc(const int64_t* sums, int32_t A, const int32_t* in, int32_t* out, int64_t V,
+                                   int32_t* dead_out, int32_t* final_label_num, void* stream);
 
 /* Image::initialization (GPU-AS/Image.cpp:350-620): selects the initial
  * seed placement, ASV_INIT_GRID or ASV_INIT_DENSITY (the grid re-placed by
