@@ -126,7 +126,8 @@ class StreamParams(ctypes.Structure):
 
 class ServeParams(ctypes.Structure):
     _fields_ = [("memory_budget", _i64), ("max_active", _i32), ("block_size", _i32), ("overlap", _i32),
-                ("lloyd_iters", _i32), ("compactness", _f32), ("angle_gate", _i32), ("honor_arrivals", _i32)]
+                ("lloyd_iters", _i32), ("compactness", _f32), ("angle_gate", _i32), ("honor_arrivals", _i32),
+                ("batch_windows", _i32)]
 
 
 class ServeRequest(ctypes.Structure):

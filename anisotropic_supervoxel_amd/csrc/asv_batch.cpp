@@ -18,6 +18,7 @@
 // with HIP events on the batch's own first iteration.
 #include "../../include/asv.h"
 #include "asv_ctx.h"
+#include "asv_host_internal.h"
 #include "asv_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -58,6 +59,7 @@ struct PassGroup {
     int grid = 0, nitems = 0, dyn = 0;
     int per_block = 0, grid_limit = 0, seg_cap = 0;   // per_block 0: planned by predicted time
     bool split = false;            // tuned: the clips' own launches are faster
+    bool ht = false;               // a clip has more seeds than the marching table has slots
 };
 
 JfaClip descriptor(const asv_ctx* c, int p)
@@ -76,6 +78,97 @@ JfaClip descriptor(const asv_ctx* c, int p)
     d.ratio = c->mult[(size_t)p] * c->ratio;
     d.unsafe = c->unsafe;
     return d;
+}
+
+// Pass groups of a set of contexts, in pass order: for each pass index p,
+// one batched group per step K over the contexts that march pass p (the
+// marching kernel applies), and one group per remaining context (passes
+// over the seeds' neighbourhoods, steps above 16, the scan-order tie).
+// Descriptors are numbered consecutively through the groups (dev_off).
+void build_groups(const std::vector<asv_ctx*>& clips, std::vector<PassGroup>* groups, size_t* n_desc)
+{
+    int pmax = 0;
+    for (const asv_ctx* c : clips) pmax = std::max(pmax, (int)c->steps.size());
+    groups->clear();
+    *n_desc = 0;
+    for (int p = 0; p < pmax; ++p) {
+        std::vector<PassGroup> row;
+        for (int i = 0; i < (int)clips.size(); ++i) {
+            const asv_ctx* c = clips[(size_t)i];
+            if (p >= (int)c->steps.size()) continue;
+            const int K = c->steps[(size_t)p];
+            // passes over the seeds' neighbourhoods alone run per clip
+            const bool march = c->pass_kind((size_t)p) == 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID &&
+                               c->d.A <= kMarchMaxIds;
+            PassGroup* g = nullptr;
+            if (march)
+                for (PassGroup& q : row)
+                    if (q.batched && q.K == K) g = &q;
+            if (!g) {
+                row.emplace_back();
+                g = &row.back();
+                g->p = p;
+                g->K = K;
+                g->batched = march;
+            }
+            g->clips.push_back(i);
+            if (march) {
+                g->desc.push_back(descriptor(c, p));
+                g->ht = g->ht || c->d.A > kMarchTableIds;
+            }
+        }
+        for (PassGroup& g : row) {
+            g.dev_off = *n_desc;
+            *n_desc += g.desc.size();
+            groups->push_back(std::move(g));
+        }
+    }
+}
+
+// One group's launches on st: the marching kernel over the group's
+// descriptors at dev_desc + dev_off, or each context's own pass.
+int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, const PassGroup& g, hipStream_t st)
+{
+    if (g.batched && !g.split) {
+        BHIP(launch_march_batch(st, g.K, dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn, g.ht));
+        return ASV_OK;
+    }
+    for (int i : g.clips) {
+        const asv_ctx* c = clips[(size_t)i];
+        const float ratio = c->mult[(size_t)g.p] * c->ratio;
+        const int32_t* in = c->labels[g.p & 1];
+        int32_t* out = c->labels[1 - (g.p & 1)];
+        switch (c->pass_kind((size_t)g.p)) {       // every iteration starts with the seed stage
+        case 1:
+            BHIP(launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, g.K, ratio, c->p.tie_rule,
+                                  c->first_pass_fill()));
+            break;
+        case 2:
+            BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
+                                   c->keys));
+            break;
+        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe)); break;
+        }
+    }
+    return ASV_OK;
+}
+
+// Seed stages, the JFA passes group by group, reductions and finalisation
+// of iteration `iter` for every context, enqueued on st.
+int enqueue_iteration(const std::vector<asv_ctx*>& clips, const std::vector<PassGroup>& groups,
+                      const JfaClip* dev_desc, int iter, hipStream_t st)
+{
+    int rc;
+    for (asv_ctx* c : clips)
+        if ((rc = asv_seed_stage(c, iter, st))) return rc;
+    for (const PassGroup& g : groups)
+        if ((rc = launch_group(clips, dev_desc, g, st))) return rc;
+    for (asv_ctx* c : clips) {
+        c->fresh = false;
+        c->cur = (int)(c->steps.size() & 1);
+        if ((rc = asv_centroids(c, iter, st))) return rc;
+    }
+    return ASV_OK;
 }
 
 }  // namespace
@@ -121,45 +214,12 @@ int upload_group(asv_batch* b, PassGroup& g)
 
 int launch_group(asv_batch* b, const PassGroup& g, hipStream_t st)
 {
-    if (g.batched && !g.split) {
-        BHIP(launch_march_batch(st, g.K, b->dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn));
-        return ASV_OK;
-    }
-    for (int i : g.clips) {
-        const asv_ctx* c = b->clips[(size_t)i];
-        const float ratio = c->mult[(size_t)g.p] * c->ratio;
-        const int32_t* in = c->labels[g.p & 1];
-        int32_t* out = c->labels[1 - (g.p & 1)];
-        switch (c->pass_kind((size_t)g.p)) {       // every iteration starts with the seed stage
-        case 1:
-            BHIP(launch_jfa_seeds(st, c->d, in, out, c->lab, c->rec, c->pos, g.K, ratio, c->p.tie_rule,
-                                  c->first_pass_fill()));
-            break;
-        case 2:
-            BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
-                                   c->keys));
-            break;
-        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe)); break;
-        }
-    }
-    return ASV_OK;
+    return launch_group(b->clips, b->dev_desc, g, st);
 }
 
-// Seed stages, the JFA passes group by group, reductions and finalisation
-// of iteration `iter` for every clip, enqueued on st.
 int enqueue_iteration(asv_batch* b, int iter, hipStream_t st)
 {
-    int rc;
-    for (asv_ctx* c : b->clips)
-        if ((rc = asv_seed_stage(c, iter, st))) return rc;
-    for (const PassGroup& g : b->groups)
-        if ((rc = launch_group(b, g, st))) return rc;
-    for (asv_ctx* c : b->clips) {
-        c->fresh = false;
-        c->cur = (int)(c->steps.size() & 1);
-        if ((rc = asv_centroids(c, iter, st))) return rc;
-    }
-    return ASV_OK;
+    return enqueue_iteration(b->clips, b->groups, b->dev_desc, iter, st);
 }
 
 // Times each batched group's candidate launch shapes (grid limit x work
@@ -178,7 +238,7 @@ int autotune(asv_batch* b)
     float total = 0.0f;
     for (PassGroup& g : b->groups) {
         if (g.batched) {
-            const int gmax = march_max_grid(g.K, g.dyn);
+            const int gmax = march_max_grid(g.K, g.dyn, g.ht);
             struct Shape {
                 int gl, pb, cap;
                 bool split;
@@ -255,36 +315,7 @@ int asv_batch_create(const asv_params* params, int32_t n, asv_batch** out)
         b->clips.push_back(c);
         b->pmax = std::max(b->pmax, (int)c->steps.size());
     }
-    // pass groups: batched per (p, K) where the marching kernel applies
-    for (int p = 0; p < b->pmax; ++p) {
-        std::vector<PassGroup> row;
-        for (int i = 0; i < n; ++i) {
-            const asv_ctx* c = b->clips[(size_t)i];
-            if (p >= (int)c->steps.size()) continue;
-            const int K = c->steps[(size_t)p];
-            // passes over the seeds' neighbourhoods alone run per clip
-            const bool march = c->pass_kind((size_t)p) == 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID &&
-                               c->d.A <= kMarchMaxIds;
-            PassGroup* g = nullptr;
-            if (march)
-                for (PassGroup& q : row)
-                    if (q.batched && q.K == K) g = &q;
-            if (!g) {
-                row.emplace_back();
-                g = &row.back();
-                g->p = p;
-                g->K = K;
-                g->batched = march;
-            }
-            g->clips.push_back(i);
-            if (march) g->desc.push_back(descriptor(c, p));
-        }
-        for (PassGroup& g : row) {
-            g.dev_off = b->n_desc;
-            b->n_desc += g.desc.size();
-            b->groups.push_back(std::move(g));
-        }
-    }
+    build_groups(b->clips, &b->groups, &b->n_desc);
     hipError_t e = hipStreamCreate(&b->st);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
@@ -396,3 +427,161 @@ int asv_batch_tuning(const asv_batch* b, int32_t* shapes, int32_t cap, float* tu
 const char* asv_batch_last_error(void) { return g_batch_err.c_str(); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// WindowBatcher: the Lloyd loops of contexts owned elsewhere (the served
+// loop's streams) run together, with no autotuning and no graphs (the set
+// of contexts changes from one call to the next).  Each context's own work
+// -- prepare, seed stages, its own passes (those over the seeds'
+// neighbourhoods), reduction and finalisation -- goes on its own stream, so
+// that work of different contexts overlaps on the device as it would with
+// one stream per context; the marching passes of all contexts sharing a
+// pass index and step go as one launch on the batch stream.  Events join a
+// context's stream and the batch stream where its work moves between them,
+// so every context's work stays in order and ends on its own stream.  The
+// marching descriptors go from a page-locked buffer to the device on the
+// batch stream; the buffers form a ring of four, each reused only after
+// the event recorded behind the last launch that read it.
+// ---------------------------------------------------------------------------
+struct asv::WindowBatcher::Impl {
+    struct Slot {
+        JfaClip* host = nullptr;      // page-locked staging of the descriptors
+        JfaClip* dev = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;    // recorded after the call's last batched launch
+        bool used = false;
+    };
+    static constexpr int kSlots = 4;
+    Slot ring[kSlots];
+    int next = 0;
+    std::vector<hipEvent_t> ev;       // per context: its stream's work so far
+    hipEvent_t ev_batch = nullptr;    // the batch stream's work so far
+
+    ~Impl()
+    {
+        for (Slot& s : ring) {
+            if (s.done) {
+                if (s.used) (void)hipEventSynchronize(s.done);
+                (void)hipEventDestroy(s.done);
+            }
+            if (s.host) (void)hipHostFree(s.host);
+            if (s.dev) (void)hipFree(s.dev);
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (ev_batch) (void)hipEventDestroy(ev_batch);
+    }
+};
+
+asv::WindowBatcher::WindowBatcher() : impl_(new Impl) {}
+
+asv::WindowBatcher::~WindowBatcher() { delete impl_; }
+
+int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, void* batch_stream)
+{
+    if (!ctxs || !streams || n < 1) return bfail(ASV_ERR_ARG, "no contexts to run");
+    Impl& im = *impl_;
+    hipStream_t bst = reinterpret_cast<hipStream_t>(batch_stream);
+    std::vector<asv_ctx*> clips(ctxs, ctxs + n);
+    std::vector<hipStream_t> own(n);
+    for (int i = 0; i < n; ++i) own[(size_t)i] = reinterpret_cast<hipStream_t>(streams[i]);
+    const int iters = clips[0]->p.lloyd_iters;
+    for (const asv_ctx* c : clips)
+        if (!c || c->p.lloyd_iters != iters)
+            return bfail(ASV_ERR_ARG, "contexts of one batched run share the Lloyd iteration count");
+    while ((int)im.ev.size() < n) {
+        hipEvent_t e = nullptr;
+        BHIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        im.ev.push_back(e);
+    }
+    if (!im.ev_batch) BHIP(hipEventCreateWithFlags(&im.ev_batch, hipEventDisableTiming));
+    int rc;
+    for (int i = 0; i < n; ++i)                   // LAB, as asv_run does
+        if ((rc = asv_prepare(clips[(size_t)i], own[(size_t)i]))) return bfail(rc, "prepare", asv_last_error());
+    std::vector<PassGroup> groups;
+    size_t n_desc = 0;
+    build_groups(clips, &groups, &n_desc);
+    for (PassGroup& g : groups)
+        if (g.batched) {
+            g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), 0, 0, 0, &g.nitems, &g.dyn);
+            if (g.grid <= 0) g.split = true;      // each context's own launch
+        }
+    Impl::Slot& sl = im.ring[im.next];
+    im.next = (im.next + 1) % Impl::kSlots;
+    if (sl.used) BHIP(hipEventSynchronize(sl.done));
+    sl.used = false;
+    if (n_desc > sl.cap) {
+        if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.dev) (void)hipFree(sl.dev);
+        sl.host = nullptr;
+        sl.dev = nullptr;
+        sl.cap = 0;
+        const size_t cap = 2 * n_desc;
+        BHIP(hipHostMalloc((void**)&sl.host, sizeof(JfaClip) * cap, hipHostMallocDefault));
+        BHIP(dev_malloc((void**)&sl.dev, sizeof(JfaClip) * cap));
+        sl.cap = cap;
+    }
+    if (!sl.done) BHIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    size_t k = 0;                                 // descriptors in dev_off order
+    for (const PassGroup& g : groups)
+        for (const JfaClip& d : g.desc) sl.host[k++] = d;
+    if (n_desc) BHIP(hipMemcpyAsync(sl.dev, sl.host, sizeof(JfaClip) * n_desc, hipMemcpyHostToDevice, bst));
+
+    // where each context's latest work is (0 its stream, 1 the batch
+    // stream), and whether the batch stream has work past ev_batch
+    std::vector<char> on_batch((size_t)n, 0);
+    bool batch_ahead = false;
+    auto to_own = [&](int i) -> int {
+        if (!on_batch[(size_t)i]) return ASV_OK;
+        if (batch_ahead) {
+            BHIP(hipEventRecord(im.ev_batch, bst));
+            batch_ahead = false;
+        }
+        BHIP(hipStreamWaitEvent(own[(size_t)i], im.ev_batch, 0));
+        on_batch[(size_t)i] = 0;
+        return ASV_OK;
+    };
+    auto to_batch = [&](int i) -> int {
+        if (on_batch[(size_t)i]) return ASV_OK;
+        BHIP(hipEventRecord(im.ev[(size_t)i], own[(size_t)i]));
+        BHIP(hipStreamWaitEvent(bst, im.ev[(size_t)i], 0));
+        on_batch[(size_t)i] = 1;
+        return ASV_OK;
+    };
+    for (int iter = 1; iter <= iters && !rc; ++iter) {
+        for (int i = 0; i < n && !rc; ++i)
+            if (!(rc = to_own(i))) rc = asv_seed_stage(clips[(size_t)i], iter, own[(size_t)i]);
+        for (const PassGroup& g : groups) {
+            if (rc) break;
+            if (g.batched && !g.split) {
+                for (int i : g.clips)
+                    if (!rc) rc = to_batch(i);
+                if (!rc) {
+                    const hipError_t e = launch_march_batch(bst, g.K, sl.dev + g.dev_off, (int)g.desc.size(), g.grid,
+                                                            g.nitems, g.dyn, g.ht);
+                    if (e != hipSuccess) rc = bfail(ASV_ERR_HIP, "marching launch", hipGetErrorString(e));
+                    batch_ahead = true;
+                }
+            } else {
+                for (int i : g.clips) {
+                    if (rc) break;
+                    PassGroup one = g;
+                    one.clips.assign(1, i);
+                    one.split = true;
+                    if (!(rc = to_own(i))) rc = launch_group(clips, sl.dev, one, own[(size_t)i]);
+                }
+            }
+        }
+        for (int i = 0; i < n && !rc; ++i) {
+            asv_ctx* c = clips[(size_t)i];
+            if ((rc = to_own(i))) break;
+            c->fresh = false;
+            c->cur = (int)(c->steps.size() & 1);
+            rc = asv_centroids(c, iter, own[(size_t)i]);
+        }
+    }
+    BHIP(hipEventRecord(sl.done, bst));           // also behind a partial enqueue
+    sl.used = true;
+    for (int i = 0; i < n; ++i)                   // every context's work ends on its stream
+        if (!rc) rc = to_own(i);
+    return rc;
+}

@@ -113,7 +113,7 @@ inline int seed_pass_kind(const std::vector<int>& steps, int64_t A, int64_t V, s
 hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
                             const SeedRec* rec, const int4* pos, int k, float ratio, int tie_rule, bool fill = true);
 // The marching pass over several clips at once (step k <= 16, lowest-id
-// tie, A <= kMarchMaxIds per clip).  march_plan fills seg, nseg and
+// tie; more than kMarchTableIds seeds share table slots, see k_jfa_march).  march_plan fills seg, nseg and
 // item_begin of the n host descriptors (per_block > 0: about per_block
 // items per resident block; 0: the segment count with the least predicted
 // time; grid_limit > 0 caps the grid, seg_cap > 0 the slices per segment),
@@ -121,11 +121,15 @@ hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, in
 // size (0: the marching pass does not apply); the descriptors then go to
 // device memory for launch_march_batch.  march_max_grid is the resident
 // block count for `dyn` dynamic LDS bytes.
-constexpr int kMarchMaxIds = 16384;
+constexpr int kMarchTableIds = 16384;       // byte-table slots (dynamic LDS <= 16 KiB)
+constexpr int kMarchMaxIds = (1 << 30) - 1;  // ids below the resolved-index tag
+// ht: some clip of the launch has more than kMarchTableIds seeds (the
+// kernel whose table slots are shared and checked).
 int march_mc(int k);
-int march_max_grid(int k, int dyn);
+int march_max_grid(int k, int dyn, bool ht = false);
 int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn);
-hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn);
+hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
+                              bool ht);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums);

@@ -1301,7 +1301,7 @@ template <> struct MarchMask<31> { using T = uint32_t; };
 template <> struct MarchMask<63> { using T = unsigned long long; };
 
 // A staged id after the claim: the compact index it resolved to, tagged
-// with this bit (ids of the marching pass stay below kMarchMaxIds < 2^30).
+// with this bit (ids of the marching pass stay below 2^30).
 constexpr int kResolved = 0x40000000;
 
 // Slices of residue r (z = r, r + K, ...) in segment g of a clip.
@@ -1313,8 +1313,8 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
     return rest <= 0 ? 0 : (rest < c.seg ? rest : c.seg);
 }
 
-// Marching persistent pass for the lowest-id rule (K <= 16, every clip's
-// A <= kMarchMaxIds), also the batched form: one launch covers one clip
+// Marching persistent pass for the lowest-id rule (K <= 16), also the
+// batched form: one launch covers one clip
 // (`one`, clips == nullptr) or the `nclips` descriptors of `clips`.
 //
 // Work items: a clip's 16 x 16 columns (tx, ty) x residues r = z mod K x
@@ -1345,7 +1345,15 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 // record staging are paid once per two slices.
 // A rebuild that still leaves more than MC candidates evaluates that slice
 // per tap from global labels and records.
-template <int K, int MC, int S>
+//
+// The byte table has kMarchTableIds slots.  HT (a launch with a clip of
+// more seeds than slots): id i takes slot i mod kMarchTableIds, so a clip
+// with more seeds than slots can have two ids of one slot in a block's
+// region; for such a clip every looked-up index is checked against the
+// candidate list after the claims, ids are not resolved during the claim,
+// and a mismatch makes the step evaluate per tap and the next step rebuild
+// (so no later mask holds a cell whose id was dropped).
+template <int K, int MC, int S, bool HT>
 __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
                                                    int nitems, int slot_words)
 {
@@ -1380,11 +1388,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     __shared__ float4 s_z[MC];                // (z, 2 m2, 2 m4, m5)
     __shared__ int s_list[kListCap];
     __shared__ int s_count;
+    __shared__ int s_coll;                    // the last step (gstep) that met a slot shared by two ids
 
     const int tid = threadIdx.x;
     for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
-    if (tid == 0) s_count = 0;
+    if (tid == 0) {
+        s_count = 0;
+        s_coll = -1;
+    }
     __syncthreads();
+    int gstep = 0;                            // steps of this block so far (uniform)
 
     const int nb = gridDim.x;                 // a multiple of 8
     const int xcd = blockIdx.x & 7, slot_in_xcd = blockIdx.x >> 3, per_xcd = nb >> 3;
@@ -1433,6 +1446,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const float ratio = c.ratio;
         const int tx = t % c.tiles_x, ty = t / c.tiles_x;
         const int x0 = tx * kTile, y0 = ty * kTile;
+        const bool hashed = HT && c.A > kMarchTableIds;   // uniform per item
         // a slab is staged in raster order over the grown tile: thread tid
         // holds elements tid, tid + 256, ... (offsets within a frame, -1 for
         // none), so every lane of every wave stages
@@ -1475,13 +1489,14 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 const int id = ids[i];
                 const int left = __builtin_amdgcn_update_dpp(-2, id, 0x111, 0xF, 0xF, false);
                 if (id >= 0 && (kResolveInClaim || id != left)) {
-                    const unsigned sh = (unsigned)(id & 3) * 8u;
-                    uint32_t old = s_slot[id >> 2];
+                    const int sid = HT ? id & (kMarchTableIds - 1) : id;
+                    const unsigned sh = (unsigned)(sid & 3) * 8u;
+                    uint32_t old = s_slot[sid >> 2];
                     unsigned byte = (old >> sh) & 0xFFu;
                     if (byte == kSlotEmpty && id != left) {
                         while (((old >> sh) & 0xFFu) == kSlotEmpty) {
                             const uint32_t nw = (old & ~(0xFFu << sh)) | (kSlotClaimed << sh);
-                            const uint32_t prev = atomicCAS(&s_slot[id >> 2], old, nw);
+                            const uint32_t prev = atomicCAS(&s_slot[sid >> 2], old, nw);
                             if (prev == old) {
                                 const int k = atomicAdd(&s_count, 1);
                                 if (k < kListCap) s_list[k] = id;
@@ -1489,14 +1504,14 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                                 // one atomic turns the claimed byte into the
                                 // index: lanes reading the word meanwhile see
                                 // either the claim or the index
-                                atomicXor(&s_slot[id >> 2], (kSlotClaimed ^ byte) << sh);
+                                atomicXor(&s_slot[sid >> 2], (kSlotClaimed ^ byte) << sh);
                                 break;
                             }
                             old = prev;
                             byte = (old >> sh) & 0xFFu;
                         }
                     }
-                    if (kResolveInClaim && byte < kSlotClaimed) ids[i] = kResolved | (int)byte;
+                    if (kResolveInClaim && !hashed && byte < kSlotClaimed) ids[i] = kResolved | (int)byte;
                 }
             }
         };
@@ -1507,8 +1522,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 const int v = ids[i];
                 if (f < RA) {
                     unsigned cix = (unsigned)MC;
-                    if (v >= kResolved) cix = (unsigned)v & 0xFFu;
-                    else if (v >= 0) cix = (s_slot[v >> 2] >> ((unsigned)(v & 3) * 8u)) & 0xFFu;
+                    if (v >= kResolved) {
+                        cix = (unsigned)v & 0xFFu;
+                    } else if (v >= 0) {
+                        const int sid = HT ? v & (kMarchTableIds - 1) : v;
+                        cix = (s_slot[sid >> 2] >> ((unsigned)(sid & 3) * 8u)) & 0xFFu;
+                        if (hashed && cix < (unsigned)MC && s_list[cix] != v) {
+                            cix = (unsigned)MC;            // the slot holds another id
+                            s_coll = gstep;
+                        }
+                    }
                     s_idx[slot * RA + f] = (uint8_t)cix;
                 }
             }
@@ -1572,8 +1595,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         auto release = [&]() {
             if (known <= kListCap) {
                 for (int e = tid; e < known; e += 256) {
-                    const int id = s_list[e];
-                    atomicOr(&s_slot[id >> 2], 0xFFu << ((unsigned)(id & 3) * 8u));
+                    const int sid = HT ? s_list[e] & (kMarchTableIds - 1) : s_list[e];
+                    atomicOr(&s_slot[sid >> 2], 0xFFu << ((unsigned)(sid & 3) * 8u));
                 }
             } else {
                 for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
@@ -1626,6 +1649,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     stage(rs, 0, known, z);
                 }
                 __syncthreads();
+                overflow = overflow || (hashed && s_coll == gstep);
                 if (!overflow) {
 #pragma unroll
                     for (int j = 0; j < S + 2; ++j) m[j] = slab_mask(j);
@@ -1648,10 +1672,15 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 stage(rs, known, cnt, z);
                 known = cnt;
                 __syncthreads();
-                m[0] = m[S];
-                m[1] = m[S + 1];
+                if (hashed && s_coll == gstep) {
+                    overflow = true;          // this step per tap, the next one rebuilds
+                    full = true;
+                } else {
+                    m[0] = m[S];
+                    m[1] = m[S + 1];
 #pragma unroll
-                for (int k = 0; k < S; ++k) m[2 + k] = slab_mask(k);
+                    for (int k = 0; k < S; ++k) m[2 + k] = slab_mask(k);
+                }
             }
             const int ns = min(S, n - step);  // slices of this step
             // the next step's new slabs (and LAB), loaded while this step is evaluated
@@ -1732,6 +1761,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             }
             z += S * K;
             step += S;
+            ++gstep;
         }
     }
 }
@@ -2036,23 +2066,34 @@ int march_s(int k)
 
 using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
 
-template <int K>
+template <int K, bool HT>
 static MarchFn march_fn(int mc, int sl)
 {
-    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2> : k_jfa_march<K, 63, 2>;
-    return mc == 31 ? k_jfa_march<K, 31, 1> : k_jfa_march<K, 63, 1>;
+    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2, HT> : k_jfa_march<K, 63, 2, HT>;
+    return mc == 31 ? k_jfa_march<K, 31, 1, HT> : k_jfa_march<K, 63, 1, HT>;
 }
 
-static MarchFn march_kernel(int k, int mc, int sl)
+// The marching kernel for step k, candidate capacity mc, slices per step sl;
+// ht: some clip of the launch has more seeds than the table has slots.
+template <bool HT>
+static MarchFn march_kernel_ht(int k, int mc, int sl)
 {
     switch (k) {
-    case 1: return march_fn<1>(mc, sl);   case 2: return march_fn<2>(mc, sl);   case 3: return march_fn<3>(mc, sl);
-    case 4: return march_fn<4>(mc, sl);   case 5: return march_fn<5>(mc, sl);   case 6: return march_fn<6>(mc, sl);
-    case 7: return march_fn<7>(mc, sl);   case 8: return march_fn<8>(mc, sl);   case 9: return march_fn<9>(mc, sl);
-    case 10: return march_fn<10>(mc, sl); case 11: return march_fn<11>(mc, sl); case 12: return march_fn<12>(mc, sl);
-    case 13: return march_fn<13>(mc, sl); case 14: return march_fn<14>(mc, sl); case 15: return march_fn<15>(mc, sl);
-    case 16: return march_fn<16>(mc, sl); default: return nullptr;
+    case 1: return march_fn<1, HT>(mc, sl);   case 2: return march_fn<2, HT>(mc, sl);
+    case 3: return march_fn<3, HT>(mc, sl);   case 4: return march_fn<4, HT>(mc, sl);
+    case 5: return march_fn<5, HT>(mc, sl);   case 6: return march_fn<6, HT>(mc, sl);
+    case 7: return march_fn<7, HT>(mc, sl);   case 8: return march_fn<8, HT>(mc, sl);
+    case 9: return march_fn<9, HT>(mc, sl);   case 10: return march_fn<10, HT>(mc, sl);
+    case 11: return march_fn<11, HT>(mc, sl); case 12: return march_fn<12, HT>(mc, sl);
+    case 13: return march_fn<13, HT>(mc, sl); case 14: return march_fn<14, HT>(mc, sl);
+    case 15: return march_fn<15, HT>(mc, sl); case 16: return march_fn<16, HT>(mc, sl);
+    default: return nullptr;
     }
+}
+
+static MarchFn march_kernel(int k, int mc, int sl, bool ht = false)
+{
+    return ht ? march_kernel_ht<true>(k, mc, sl) : march_kernel_ht<false>(k, mc, sl);
 }
 
 // Dynamic LDS of the marching pass: the byte-per-id table for ids < amax,
@@ -2065,18 +2106,18 @@ static int march_dyn_bytes(int amax)
 
 // Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
 // 8), cached per step, capacity and table size.
-int march_max_grid(int k, int dyn)
+int march_max_grid(int k, int dyn, bool ht)
 {
-    static int cache[17][2][2][kMarchMaxIds / 1024 + 1];
-    if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchMaxIds) return 0;
+    static int cache[17][2][2][2][kMarchTableIds / 1024 + 1];
+    if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchTableIds) return 0;
     const int mc = march_mc(k), sl = march_s(k);
-    int& slot = cache[k][mc == 63][sl == 2][dyn / 1024];
+    int& slot = cache[k][mc == 63][sl == 2][ht][dyn / 1024];
     int g = __atomic_load_n(&slot, __ATOMIC_RELAXED);
     if (g == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl), 256,
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl, ht), 256,
                                                            (size_t)dyn);
         g = cus * (per_cu > 0 ? per_cu : 1);
         g = (g / 8) * 8;
@@ -2120,8 +2161,8 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
     int amax = 1;
     for (int i = 0; i < n; ++i) amax = c[i].A > amax ? c[i].A : amax;
     if (amax > kMarchMaxIds) return 0;
-    const int lds = march_dyn_bytes(amax);
-    int gmax = march_max_grid(k, lds);
+    const int lds = march_dyn_bytes(amax < kMarchTableIds ? amax : kMarchTableIds);
+    int gmax = march_max_grid(k, lds, amax > kMarchTableIds);
     if (grid_limit > 0 && grid_limit < gmax) gmax = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
     const int64_t pmax = gmax / 8;                       // resident blocks per XCD
     int64_t columns = 0;
@@ -2156,9 +2197,9 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
 }
 
 hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
-                        int dyn)
+                        int dyn, bool ht)
 {
-    MarchFn fn = march_kernel(k, march_mc(k), march_s(k));
+    MarchFn fn = march_kernel(k, march_mc(k), march_s(k), ht);
     if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
     JfaClip a0 = one;
     const JfaClip* a1 = dev;
@@ -2167,10 +2208,11 @@ hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip
     return hipLaunchKernel((const void*)fn, dim3((unsigned)grid), dim3(256), args, (size_t)dyn, st);
 }
 
-hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn)
+hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
+                              bool ht)
 {
     JfaClip none{};
-    return launch_march(st, k, none, dev, n, grid, nitems, dyn);
+    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht);
 }
 
 template <int K>
@@ -2182,7 +2224,7 @@ static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, 
         JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe};
         int nitems = 0, dyn = 0;
         const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
-        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn);
+        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds);
     }
     int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,

@@ -6,10 +6,19 @@
 // A dispatcher admits requests in arrival order as a stream slot and device
 // memory (the stream's resident window footprint) free up; a request that
 // does not fit waits while later, smaller ones may go first.  Each admitted
-// stream runs on a worker thread with its own HIP stream, so windows of
-// different streams overlap on the device.  Per window the loop records
-// when it became ready (frames present and the previous window done), when
-// its GPU work was enqueued and when its labels were back.
+// stream runs on a worker thread with its own HIP stream.  With
+// batch_windows, a worker sets up its window (seeds, frames) on its stream
+// and hands the window's context and stream to a batcher thread, which
+// enqueues the Lloyd loops of every window handed over since its last call
+// together: each window's own kernels on the window's stream, the marching
+// passes of all of them as one launch per pass index and step on the
+// batcher's stream, joined by events; the worker then reads back on its
+// stream.  Windows that become ready while a batch is on the device join
+// the next one, and the workers' set-up and read-back overlap it.
+// Without it each worker runs its window's loop on its own stream.  Per
+// window the loop records when it became ready (frames present and the
+// previous window done), when its GPU work was enqueued and when its labels
+// were back.
 #include "../../include/asv.h"
 #include "asv_host_internal.h"
 
@@ -72,6 +81,7 @@ void asv_serve_default_params(asv_serve_params* p)
     p->compactness = 6.0f;
     p->angle_gate = 0;
     p->honor_arrivals = 1;
+    p->batch_windows = 0;      // measured slower than per-stream launches on the served workload
 }
 
 int64_t asv_serve_footprint(const asv_serve_params* sp, const asv_serve_request* r)
@@ -138,6 +148,72 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     struct WindowOut {
         int32_t* p = nullptr;
         size_t cap = 0;
+    };
+
+    // Windows handed to the batcher: the context whose loop to run, the
+    // worker's stream, and the outcome of the enqueue.
+    struct Ticket {
+        asv_ctx* ctx;
+        hipStream_t st;
+        int rc;
+        bool finished;
+    };
+    std::mutex bmu;
+    std::condition_variable bcv;
+    std::deque<Ticket*> bqueue;
+    bool bstop = false;
+    auto batcher = [&]() {
+        (void)hipSetDevice(dev);
+        hipStream_t bst = nullptr;
+        const hipError_t se = hipStreamCreate(&bst);
+        asv::WindowBatcher wb;
+        std::vector<Ticket*> take;
+        std::vector<asv_ctx*> ctxs;
+        std::vector<void*> streams;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(bmu);
+                bcv.wait(lk, [&] { return bstop || !bqueue.empty(); });
+                if (bqueue.empty()) break;
+                take.assign(bqueue.begin(), bqueue.end());
+                bqueue.clear();
+            }
+            int rc = se == hipSuccess ? ASV_OK : ASV_ERR_HIP;
+            ctxs.clear();
+            streams.clear();
+            for (Ticket* t : take) {
+                ctxs.push_back(t->ctx);
+                streams.push_back(t->st);
+            }
+            if (!rc) rc = wb.run(ctxs.data(), streams.data(), (int)ctxs.size(), bst);
+            {
+                std::lock_guard<std::mutex> g(bmu);
+                for (Ticket* t : take) {
+                    t->rc = rc;
+                    t->finished = true;
+                }
+            }
+            bcv.notify_all();
+        }
+        if (bst) {
+            (void)hipStreamSynchronize(bst);
+            (void)hipStreamDestroy(bst);
+        }
+    };
+    // One window's Lloyd loop through the batcher, enqueued (behind the
+    // window's set-up) on the worker's stream and the batcher's.
+    auto run_batched = [&](asv_ctx* c, hipStream_t st) -> int {
+        Ticket t{c, st, ASV_OK, false};
+        {
+            std::lock_guard<std::mutex> g(bmu);
+            bqueue.push_back(&t);
+        }
+        bcv.notify_all();
+        {
+            std::unique_lock<std::mutex> lk(bmu);
+            bcv.wait(lk, [&] { return t.finished; });
+        }
+        return t.rc;
     };
     auto process = [&](int ri, hipStream_t st, WindowOut& wo) -> int {
         const asv_serve_request& r = reqs[ri];
@@ -212,7 +288,13 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             w.ready_ms = ready;
             w.start_ms = ms_since(t0);
             int32_t done = 0;
-            rc = asv_stream_block_into(s, b, rgb, fwd, bwd, wo.p, &done, st);
+            if (sp->batch_windows) {
+                rc = asv::stream_block_begin(s, b, rgb, fwd, bwd, st);
+                if (!rc) rc = run_batched(asv::stream_ctx(s), st);
+                if (!rc) rc = asv::stream_block_end(s, wo.p, &done, st);
+            } else {
+                rc = asv_stream_block_into(s, b, rgb, fwd, bwd, wo.p, &done, st);
+            }
             w.end_ms = ms_since(t0);
             int32_t info[6];
             asv_stream_block_info(s, info);
@@ -276,6 +358,8 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         if (st) (void)hipStreamDestroy(st);
     };
 
+    std::thread batch_thread;
+    if (sp->batch_windows) batch_thread = std::thread(batcher);
     std::vector<std::thread> pool;
     for (int i = 0; i < sp->max_active; ++i) pool.emplace_back(worker);
 
@@ -336,6 +420,14 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
     }
     cv.notify_all();
     for (auto& th : pool) th.join();
+    if (batch_thread.joinable()) {
+        {
+            std::lock_guard<std::mutex> g(bmu);
+            bstop = true;
+        }
+        bcv.notify_all();
+        batch_thread.join();
+    }
     for (const Idle& e : idle) asv_destroy(e.c);
     if (n_windows) *n_windows = nwin;
     if (wall_ms) *wall_ms = ms_since(t0);

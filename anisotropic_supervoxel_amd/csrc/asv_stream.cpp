@@ -92,6 +92,10 @@ struct asv_stream {
     asv_ctx* ctx = nullptr;
     int depth_cap = 0, seed_cap = 0, prev_depth = 0;
     uint64_t checksum = 0;                // of the frames finished so far
+    // the block set up by stream_block_begin and not yet finished
+    int pending = -1;
+    int pending_A = 0, pending_budget = 0, pending_nkept = 0;
+    std::vector<int> pending_map;         // its block2all
     ~asv_stream()
     {
         if (ctx) asv_destroy(ctx);
@@ -223,16 +227,23 @@ int asv_stream_block(asv_stream* s, int32_t block, const uint8_t* rgb, const flo
                                  frames_done, stream);
 }
 
-int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
-                          int32_t* out, int32_t* frames_done, void* stream)
+}  // extern "C"
+
+// First half of a block (prepare_data_seeds, streamGPU-AS/Render.cpp:348-470):
+// the new-seed budget and grid, kept ++ new seeds and the block's map to
+// global ids, the resident context reseeded, the overlap frames moved on the
+// device and the new frames uploaded, all on `stream`.  The Lloyd loop is
+// the caller's (asv_run, or a batched run over several streams' contexts).
+int asv::stream_block_begin(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
+                            void* stream)
 {
-    if (!s || !rgb || !fwd || !bwd || !out) return sfail(ASV_ERR_ARG, "null argument");
+    if (!s || !rgb || !fwd || !bwd) return sfail(ASV_ERR_ARG, "null argument");
     const int nblocks = (int)s->intervals.size() / 2;
+    if (s->pending >= 0) return sfail(ASV_ERR_STATE, "the previous block is not finished");
     if (block != s->next_block || block >= nblocks) return sfail(ASV_ERR_STATE, "blocks must be processed in order");
-    const int W = s->p.width, H = s->p.height, B = s->p.block_size, O = s->p.overlap, S = B - O;
+    const int W = s->p.width, H = s->p.height, O = s->p.overlap;
     const int start = s->intervals[2 * block], end = s->intervals[2 * block + 1];
     const int Db = end - start;
-    const bool last = block == nblocks - 1;
 
     // new-seed budget (:354-356, :394)
     int budget = (block == 0 && nblocks == 1) ? s->seeds100 * Db / 100 : s->seeds100 * (Db - O) / 100;
@@ -284,25 +295,49 @@ int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, cons
         if (!rc) rc = asv_reseed(s->ctx, xyz.data(), A, sgrid, Db, stream);
     }
     if (rc) return rc;
+    rc = asv_upload_frames_range(s->ctx, resident, Db - resident, rgb + (size_t)resident * plane * 3,
+                                 fwd + (size_t)resident * plane * 2, bwd + (size_t)resident * plane * 2, stream);
+    if (rc) return rc;
+    s->pending = block;
+    s->pending_A = A;
+    s->pending_budget = budget;
+    s->pending_nkept = nkept;
+    s->pending_map.swap(block2all);
+    return ASV_OK;
+}
+
+asv_ctx* asv::stream_ctx(asv_stream* s) { return s ? s->ctx : nullptr; }
+
+// Second half (block_segment's readback and post_and_next,
+// streamGPU-AS/Render.cpp:470-569), after the block's Lloyd loop on
+// `stream`: check_dead_seeds (streamGPU-AS/Image.cpp:1534-1579) on the
+// device, the block2all stitch (:559-563, an id absent from the map reads as
+// 0 as std::map operator[] gives) with only the finished frames copied to
+// `out`, and the seeds kept for the next block.
+int asv::stream_block_end(asv_stream* s, int32_t* out, int32_t* frames_done, void* stream)
+{
+    if (!s || !out) return sfail(ASV_ERR_ARG, "null argument");
+    if (s->pending < 0) return sfail(ASV_ERR_STATE, "no block has begun");
+    const int block = s->pending;
+    s->pending = -1;
+    const int nblocks = (int)s->intervals.size() / 2;
+    const int W = s->p.width, H = s->p.height, B = s->p.block_size, O = s->p.overlap, S = B - O;
+    const int start = s->intervals[2 * block], end = s->intervals[2 * block + 1];
+    const int Db = end - start;
+    const bool last = block == nblocks - 1;
+    const int A = s->pending_A, nkept = s->pending_nkept;
+    const size_t plane = (size_t)W * H;
     asv_ctx* ctx = s->ctx;
     std::vector<int64_t> sums((size_t)(A + 1) * 8);
-    rc = asv_upload_frames_range(ctx, resident, Db - resident, rgb + (size_t)resident * plane * 3,
-                                 fwd + (size_t)resident * plane * 2, bwd + (size_t)resident * plane * 2, stream);
-    if (!rc) rc = asv_run(ctx, stream);
-    if (!rc) rc = asv_download_sums(ctx, sums.data(), stream);
+    int rc = asv_download_sums(ctx, sums.data(), stream);
     if (rc) return rc;
     s->prev_depth = Db;
     const int done_frames = last ? Db : Db - O;
-    int32_t* dst = out;
-
-    // check_dead_seeds (streamGPU-AS/Image.cpp:1534-1579) on the device, then
-    // the block2all stitch (:559-563), where an id absent from the map reads
-    // as 0 (std::map operator[]); only the stitched frames come back.
     int dead = 0, nfin = 0;
     rc = asv_postprocess_device(ctx, sums.data(), &dead, &nfin, stream);
     uint64_t ck = 0;
     if (!rc)
-        rc = asv_download_labels_mapped(ctx, block2all.data(), A, done_frames, dst, (int64_t)start * (int64_t)plane,
+        rc = asv_download_labels_mapped(ctx, s->pending_map.data(), A, done_frames, out, (int64_t)start * (int64_t)plane,
                                         &ck, stream);
     if (rc) return rc;
     std::vector<KeptSeed> alive;
@@ -329,14 +364,27 @@ int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, cons
         }
     }
     s->kept.swap(next);
-    s->prev_map.swap(block2all);
+    s->prev_map.swap(s->pending_map);
     s->checksum += ck;
     s->last_A = A;
     s->last_dead = dead;
-    s->last_budget = budget;
+    s->last_budget = s->pending_budget;
     s->next_block = block + 1;
     if (frames_done) *frames_done = done_frames;
     return ASV_OK;
+}
+
+extern "C" {
+
+int asv_stream_block_into(asv_stream* s, int32_t block, const uint8_t* rgb, const float* fwd, const float* bwd,
+                          int32_t* out, int32_t* frames_done, void* stream)
+{
+    if (!s || !rgb || !fwd || !bwd || !out) return sfail(ASV_ERR_ARG, "null argument");
+    int rc = asv::stream_block_begin(s, block, rgb, fwd, bwd, stream);
+    if (!rc) rc = asv_run(s->ctx, stream);
+    if (!rc) rc = asv::stream_block_end(s, out, frames_done, stream);
+    else s->pending = -1;
+    return rc;
 }
 
 // Whole video through the blocks in order (Render::run of the streaming

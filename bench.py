@@ -90,15 +90,19 @@ def served(n_streams, actives, seed):
     Frames are generated up front into page-locked host memory.  Runs the
     plain per-window loop (one stream at a time, no overlap) and the served
     loop (each of `actives` concurrent-stream limits, admission by
-    footprint), and checks that every stream's stitched-label checksum is
-    the same in every mode.  Both loops deliver each window's finished
-    labels to host memory and keep the checksum on the device."""
+    footprint), each with every stream's own launches on its own HIP stream
+    (the default) and with the windows' Lloyd loops batched across the
+    concurrent streams, and checks that every stream's stitched-label
+    checksum is the same in every mode.  Both loops deliver each window's finished labels to
+    host memory and keep the checksum on the device."""
     from anisotropic_supervoxel_amd import serve
     reqs = serve.make_requests(n_streams, rate_per_s=40.0, seed=seed)
     serve.attach_frames(reqs)
     out = {}
     hashes = {}
-    modes = [("plain", dict(max_active=1, honor_arrivals=0))] + [(f"served{a}", dict(max_active=a)) for a in actives]
+    modes = [("plain", dict(max_active=1, honor_arrivals=0, batch_windows=0))]
+    modes += [(f"served{a}", dict(max_active=a, batch_windows=0)) for a in actives]
+    modes += [(f"batched{a}", dict(max_active=a, batch_windows=1)) for a in actives]
     for name, kw in modes:
         res, w, wall = serve.run(serve.default_params(**kw), reqs)
         s = serve.summarize(res, w, wall)
@@ -111,8 +115,12 @@ def served(n_streams, actives, seed):
                     "1000 px per frame, windows 10 stride 5, 5 Lloyd iterations",
         "plain_per_window_loop": out["plain"],
         "served": [dict(max_active=a, **out[f"served{a}"],
-                        served_over_plain=round(out[f"served{a}"]["mvoxel_iters_per_s"] / plain, 3)) for a in actives],
-        "labels_identical_alone_and_in_mix": all(hashes[f"served{a}"] == hashes["plain"] for a in actives),
+                        served_over_plain=round(out[f"served{a}"]["mvoxel_iters_per_s"] / plain, 3))
+                   for a in actives],
+        "served_batched_windows": [dict(max_active=a, **out[f"batched{a}"],
+                                        served_over_plain=round(out[f"batched{a}"]["mvoxel_iters_per_s"] / plain, 3))
+                                   for a in actives],
+        "labels_identical_alone_and_in_mix": all(hashes[m] == hashes["plain"] for m in hashes),
         "voxel_iters": int(sum(r["voxel_iters"] for r in res)),
     }
 
@@ -170,6 +178,47 @@ def batched(n_clips, steps):
     }
 
 
+def cpu_baselines(config3_all_cores=None):
+    """The CPU oracle (C, OpenMP) timed on the plain configurations, on all
+    host cores (16 at most) and on one: configs 1 and 2 in full on all
+    cores, config 1 in full on one; config 2 on its first 10 frames and
+    config 3 on its first 5 on one core (K scaled to keep the grid step and
+    schedule); config 3 on all cores is the cpu_baseline sample, passed in.
+    Each row says what it ran.  The oracle evaluates the metric of every
+    voxel, which dominates its single-thread time."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import anisotropic_supervoxel_amd as asv
+    import oracle as orc
+    allc = min(16, os.cpu_count() or 1)
+    rows = []
+    plan = [("plain-160x120x30-K500", None, allc), ("plain-160x120x30-K500", None, 1),
+            ("plain-320x240x85-K2000", None, allc), ("plain-320x240x85-K2000", 10, 1),
+            ("plain-640x480x200-K10000", 5, 1)]
+    videos = {}
+    for name, frames, threads in plan:
+        seed, W, H, D, ell, vmax, sigma, K, iters = WORKLOADS[name]
+        if name not in videos:
+            videos[name] = asv.synth_video(seed, W, H, D, ell, vmax, sigma)
+        rgb, fwd, bwd, _ = videos[name]
+        f = frames or D
+        Ks = K if f == D else max(1, K * f // D)
+        orc.load().orc_set_threads(threads)
+        t0 = time.perf_counter()
+        orc.run(np.ascontiguousarray(rgb[:f]), np.ascontiguousarray(fwd[:f]), np.ascontiguousarray(bwd[:f]), Ks, iters)
+        dt = time.perf_counter() - t0
+        rows.append({"workload": name, "frames": f, "K": Ks, "cores": threads, "wall_s": round(dt, 2),
+                     "mvoxel_iters_per_s": round(W * H * f * iters / dt / 1e6, 2),
+                     "sample": "full clip" if f == D else f"first {f} of {D} frames, K scaled"})
+        print(f"cpu baseline: {name} frames {f} cores {threads}: {dt:.1f} s", file=sys.stderr, flush=True)
+    orc.load().orc_set_threads(allc)
+    if config3_all_cores:
+        c = config3_all_cores
+        rows.insert(4, {"workload": "plain-640x480x200-K10000", "cores": c["cores"],
+                        "mvoxel_iters_per_s": round(c["value"], 2), "sample": c["sample"]})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +227,8 @@ def main():
     ap.add_argument("--workload", default="plain-640x480x200-K10000", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=70)
+    ap.add_argument("--no-cpu-table", action="store_true",
+                    help="skip the oracle timings on configs 1-3 (all cores and one)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-streams", type=int, default=24)
     ap.add_argument("--served-active", default="4,6,8", help="comma-separated concurrent-stream limits")
@@ -291,6 +342,8 @@ def main():
         out["served"] = served(args.served_streams, [int(a) for a in args.served_active.split(",")], seed=5)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(seed, W, H, D, ell, vmax, sigma, K, iters, args.cpu_frames)
+    if not args.no_cpu_table:
+        out["cpu_baselines"] = cpu_baselines(out["cpu_baseline"])
     print(json.dumps(out), flush=True)
 
 

@@ -385,6 +385,10 @@ typedef struct asv_serve_params {
     float   compactness;        /* 6 */
     int32_t angle_gate;         /* 0 */
     int32_t honor_arrivals;     /* 1: a request is eligible from arrival_ms of wall time */
+    int32_t batch_windows;      /* 1: the windows of concurrent streams run their Lloyd loops
+                                   together, one marching launch per pass index and step over all
+                                   of them; 0 (default): each stream runs its own launches on its
+                                   own HIP stream (measured faster on the served workload) */
 } asv_serve_params;
 
 typedef struct asv_serve_request {

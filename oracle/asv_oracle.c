@@ -61,6 +61,20 @@
 
 #define ORC_API __attribute__((visibility("default")))
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the OpenMP loops (the CPU baseline times all cores and one). */
+ORC_API void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------ */
 /* a2: RGB -> LAB  (GPU-AS/shaders/RGB2LAB.fs:11-79)                          */
 /* ------------------------------------------------------------------------ */

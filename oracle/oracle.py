@@ -65,6 +65,7 @@ def load():
             "orc_seed_metric": (None, [vp, vp, vp, i, i, i, i, i, i, i, vp]),
             "orc_angle_gate": (i, [f, f, f, f]),
             "orc_run_seeded": (i, [vp, vp, vp, vp, vp, i, i, vp, vp, vp]),
+            "orc_set_threads": (None, [i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)

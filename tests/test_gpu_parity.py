@@ -198,12 +198,37 @@ def test_exact_clamp_path_bit_exact(have_gpu, seed, shape, K, iters):
 
 
 def test_many_seeds_hash_table_path(have_gpu):
-    """More seeds than the direct-mapped LDS id table holds (A > 16384):
-    the tiles use the hashed candidate table."""
+    """More seeds than the marching pass's byte table has slots (A > 16384):
+    ids share slots (i mod 16384) and every lookup is checked."""
     rgb, fwd, bwd, _ = asv.synth_video(12, 640, 480, 48, 20, 3.0, 5.0)
     got = _clip_run(rgb, fwd, bwd, 18000, 2)
     assert got["A"] > 16384
     _assert_run_equal(got, rgb, fwd, bwd, 18000, 2)
+
+
+def test_shared_table_slots_collide_everywhere(have_gpu):
+    """32768 seeds on a 32 x 32 x 32 lattice, numbered so that x-neighbours
+    are ids m and m + 16384, which share a slot of the marching pass's byte
+    table: nearly every block meets a slot held by another id, evaluates
+    that step per tap and rebuilds.  Labels and sums bit-exact against the
+    oracle."""
+    W, H, D = 128, 128, 64
+    q = np.arange(32 * 32 * 32)
+    i, j, k = q % 32, (q // 32) % 32, q // 1024
+    lattice = np.stack([4 * i + 1, 4 * j + 1, 2 * k], axis=1).astype(np.int32)
+    ids = np.where(q % 2 == 0, q // 2, q // 2 + 16384)
+    xyz = np.empty_like(lattice)
+    xyz[ids] = lattice
+    rgb, fwd, bwd, _ = asv.synth_video(24, W, H, D, 10, 2.0, 4.0)
+    s, iters = 3, 2
+    p = asv.default_params(W, H, D, len(xyz), lloyd_iters=iters)
+    with asv.Clip(p, seeds_xyz=xyz, grid_step=s) as c:
+        c.upload(rgb, fwd, bwd)
+        c.run()
+        labels, sums = c.labels(), c.sums()
+    want, wsums, _ = orc.run_seeded(rgb, fwd, bwd, xyz, s, iters)
+    assert int((labels != want).sum()) == 0
+    assert np.array_equal(sums[:, :7], wsums)
 
 
 def test_config1_bit_exact(have_gpu):

@@ -59,6 +59,20 @@ def test_served_mix_equals_alone_and_oracle(have_gpu):
 
 
 @pytest.mark.gpu
+def test_batched_windows_equal_unbatched(have_gpu):
+    """The windows of concurrent streams run through shared batched launches
+    (batch_windows = 1) give every stream the checksum it gets when each
+    stream runs its own launches (the default), at several concurrencies."""
+    reqs = _small_requests()
+    want = {r["stream_id"]: r["labels_hash"] for r in serve.run(
+        serve.default_params(max_active=3, batch_windows=0, honor_arrivals=0), reqs)[0]}
+    for active in (1, 3, 6):
+        res, windows, _ = serve.run(serve.default_params(max_active=active, batch_windows=1, honor_arrivals=0), reqs)
+        assert all(r["status"] == 0 for r in res)
+        assert {r["stream_id"]: r["labels_hash"] for r in res} == want, active
+
+
+@pytest.mark.gpu
 def test_failures_end_only_their_own_request(have_gpu):
     """A malformed request (width 1), a request cancelled after two windows,
     one cancelled by its flag before it starts, and one whose device
@@ -199,6 +213,9 @@ def test_config5_resolutions_alone_mix_oracle(have_gpu):
     assert all(r["status"] == 0 for r in mix)
     assert max(w["seeds"] for w in windows if w["stream_id"] == 1) > 16384
     by_id = {r["stream_id"]: r for r in mix}
+    # the same mix with the windows' loops batched into shared launches
+    mixb, _, _ = serve.run(serve.default_params(max_active=3, honor_arrivals=0, batch_windows=1), reqs)
+    assert {r["stream_id"]: r["labels_hash"] for r in mixb} == {r["stream_id"]: r["labels_hash"] for r in mix}
     for r in reqs:
         alone, _, _ = serve.run(serve.default_params(max_active=1, honor_arrivals=0), [r])
         assert alone[0]["labels_hash"] == by_id[r["stream_id"]]["labels_hash"], r

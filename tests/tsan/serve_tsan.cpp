@@ -90,6 +90,29 @@ hipError_t hipHostFree(void* p)
     std::free(p);
     return hipSuccess;
 }
+// Events and stream waits: the stand-in "device" work runs synchronously on
+// the calling thread, so recording and waiting are bookkeeping only.
+hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned)
+{
+    *e = reinterpret_cast<hipEvent_t>(new int(0));
+    return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e)
+{
+    delete reinterpret_cast<int*>(e);
+    return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t)
+{
+    must(e != nullptr, "event exists");
+    return hipSuccess;
+}
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t e, unsigned)
+{
+    must(e != nullptr, "event exists");
+    return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 
 // ---- device-side entry points ---------------------------------------------
 void asv_default_params(asv_params* p, int32_t w, int32_t h, int32_t d, int32_t k)
@@ -268,6 +291,24 @@ int asv_download_labels_mapped(asv_ctx* c, const int32_t* table, int32_t n_table
 
 }  // extern "C"
 
+// The batched Lloyd loops: each context's stand-in run, on the batcher's
+// thread (a context handed over while its worker still touched it would
+// race here).
+struct asv::WindowBatcher::Impl {
+    int calls = 0;
+};
+asv::WindowBatcher::WindowBatcher() : impl_(new Impl) {}
+asv::WindowBatcher::~WindowBatcher() { delete impl_; }
+int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, void*)
+{
+    ++impl_->calls;
+    for (int i = 0; i < n; ++i) {
+        const int rc = asv_run(ctxs[i], streams[i]);
+        if (rc) return rc;
+    }
+    return ASV_OK;
+}
+
 bool asv::ctx_reusable(const asv_ctx* c, const asv_params* p, int32_t seeds)
 {
     check_ctx(c);
@@ -310,10 +351,13 @@ std::vector<asv_serve_request> requests(std::vector<int32_t>& cancel_flags)
     return r;
 }
 
+int g_batch = 1;                                  // batch_windows of the runs below
+
 int run(const std::vector<asv_serve_request>& reqs, int max_active, int64_t budget, std::vector<asv_serve_result>& res)
 {
     asv_serve_params sp;
     asv_serve_default_params(&sp);
+    sp.batch_windows = g_batch;
     sp.max_active = max_active;
     sp.memory_budget = budget;
     sp.honor_arrivals = 1;
@@ -353,7 +397,8 @@ int main()
     mix[8].width = 64;                                  // the only context of at least 64 x 32 x 10 voxels
     mix[8].height = 32;
     g_fail_from.store((int64_t)64 * 32 * 10 * 31);
-    for (int rep = 0; rep < 3; ++rep) {
+    for (int rep = 0; rep < 4; ++rep) {
+        g_batch = rep & 1;                              // both forms of the loop
         std::vector<asv_serve_result> res;
         must(run(mix, 4, 3 * fp_max, res) == ASV_OK, "mix run");
         for (size_t i = 0; i < mix.size(); ++i) {
