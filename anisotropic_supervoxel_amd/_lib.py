@@ -47,6 +47,8 @@ SIGNATURES = {
     "asv_download_sums": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_enable_trace": (ctypes.c_int, [_vp, _i32]),
     "asv_force_exact_clamp": (ctypes.c_int, [_vp, _i32]),
+    "asv_enable_march_stats": (ctypes.c_int, [_vp, _i32]),
+    "asv_download_march_stats": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_debug_fail_alloc": (ctypes.c_int, [_i64]),
     "asv_download_trace": (ctypes.c_int, [_vp, _vp, _vp]),
     "asv_download_lab": (ctypes.c_int, [_vp, _vp, _vp]),

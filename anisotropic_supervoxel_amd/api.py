@@ -280,6 +280,28 @@ class Clip:
         """Every later run takes the marching pass's exact per-tap path."""
         check(self.lib.asv_force_exact_clamp(self.h, 1 if on else 0))
 
+    def enable_march_stats(self, on=True):
+        """Count the marching passes' steps in every later run (on zeroes
+        the counters)."""
+        check(self.lib.asv_enable_march_stats(self.h, 1 if on else 0))
+
+    def march_stats(self, by_step=False):
+        """Marching-step counters summed since they were enabled: steps,
+        wide steps (candidate lists evaluated in chunks), their chunks and
+        per-tap steps; for the steps within one candidate mask, candidate
+        evaluations, wave rounds (each 64-voxel wave-slice's busiest voxel)
+        and wave-slices; by_step adds the same per JFA step size."""
+        out = np.zeros((16, 8), np.uint64)
+        check(self.lib.asv_download_march_stats(self.h, ptr(out), self.stream))
+        names = ("steps", "wide", "chunks", "per_tap", "candidates", "wave_rounds", "wave_slices")
+
+        def row(r):
+            return {n: int(r[i]) for i, n in enumerate(names)}
+        total = row(out.sum(axis=0))
+        if by_step:
+            total["by_step"] = {k + 1: row(out[k]) for k in range(16) if out[k, 0]}
+        return total
+
     # -- outputs -------------------------------------------------------------
     def labels(self):
         out = np.empty(self.shape, np.int32)

@@ -60,6 +60,7 @@ struct PassGroup {
     int per_block = 0, grid_limit = 0, seg_cap = 0;   // per_block 0: planned by predicted time
     bool split = false;            // tuned: the clips' own launches are faster
     bool ht = false;               // a clip has more seeds than the marching table has slots
+    bool counted = false;          // a clip keeps marching-step counters
 };
 
 JfaClip descriptor(const asv_ctx* c, int p)
@@ -77,6 +78,7 @@ JfaClip descriptor(const asv_ctx* c, int p)
     d.tiles_y = (c->d.H + 15) / 16;
     d.ratio = c->mult[(size_t)p] * c->ratio;
     d.unsafe = c->unsafe;
+    d.stats = c->march_stats;
     return d;
 }
 
@@ -115,6 +117,7 @@ void build_groups(const std::vector<asv_ctx*>& clips, std::vector<PassGroup>* gr
             if (march) {
                 g->desc.push_back(descriptor(c, p));
                 g->ht = g->ht || c->d.A > kMarchTableIds;
+                g->counted = g->counted || c->march_stats != nullptr;
             }
         }
         for (PassGroup& g : row) {
@@ -130,7 +133,8 @@ void build_groups(const std::vector<asv_ctx*>& clips, std::vector<PassGroup>* gr
 int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, const PassGroup& g, hipStream_t st)
 {
     if (g.batched && !g.split) {
-        BHIP(launch_march_batch(st, g.K, dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn, g.ht));
+        BHIP(launch_march_batch(st, g.K, dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn, g.ht,
+                                g.counted));
         return ASV_OK;
     }
     for (int i : g.clips) {
@@ -147,7 +151,9 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
             BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
                                    c->keys));
             break;
-        default: BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe)); break;
+        default:
+            BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe, c->march_stats));
+            break;
         }
     }
     return ASV_OK;
@@ -557,7 +563,7 @@ int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, v
                     if (!rc) rc = to_batch(i);
                 if (!rc) {
                     const hipError_t e = launch_march_batch(bst, g.K, sl.dev + g.dev_off, (int)g.desc.size(), g.grid,
-                                                            g.nitems, g.dyn, g.ht);
+                                                            g.nitems, g.dyn, g.ht, g.counted);
                     if (e != hipSuccess) rc = bfail(ASV_ERR_HIP, "marching launch", hipGetErrorString(e));
                     batch_ahead = true;
                 }

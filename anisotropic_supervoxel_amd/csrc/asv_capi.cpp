@@ -171,7 +171,7 @@ void free_ctx(asv_ctx* c)
 {
     void* ptrs[] = {c->rgb, c->fwd, c->bwd, c->lut, c->lab, c->seed_own, c->seed_need,
                     c->labels[0], c->labels[1], c->grid, c->pos, c->slab, c->rec, c->sums[0], c->sums[1],
-                    c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys, c->unsafe};
+                    c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys, c->unsafe, c->march_stats};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     if (c->sums_host) (void)hipHostFree(c->sums_host);
@@ -656,7 +656,9 @@ hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
     case 2:
         return launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], c->steps[1], ratio,
                                  c->p.tie_rule, c->keys);
-    default: return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule, c->unsafe);
+    default:
+        return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule, c->unsafe,
+                          c->march_stats);
     }
 }
 
@@ -1090,6 +1092,39 @@ int asv_force_exact_clamp(asv_ctx* c, int32_t on)
 {
     if (!c) return fail(ASV_ERR_ARG, "null context");
     c->force_exact = on != 0;
+    return ASV_OK;
+}
+
+// Marching-step counters (asv_enable_march_stats): a row of 8 words (7
+// used) per step size 1..16.
+constexpr int kMarchStatWords = 16 * 8;
+
+int asv_enable_march_stats(asv_ctx* c, int32_t on)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (on && !c->march_stats) {
+        hipError_t e = asv::dev_malloc((void**)&c->march_stats, kMarchStatWords * sizeof(unsigned long long));
+        if (e != hipSuccess) { c->march_stats = nullptr; return fail(ASV_ERR_NOMEM, "counter allocation failed"); }
+        c->bytes += kMarchStatWords * 8;
+    }
+    if (on) {
+        HIP_TRY(hipMemsetAsync(c->march_stats, 0, kMarchStatWords * sizeof(unsigned long long), nullptr));
+        HIP_TRY(hipStreamSynchronize(nullptr));
+    } else if (c->march_stats) {
+        HIP_TRY(hipDeviceSynchronize());
+        (void)hipFree(c->march_stats);
+        c->march_stats = nullptr;
+        c->bytes -= kMarchStatWords * 8;
+    }
+    return ASV_OK;
+}
+
+int asv_download_march_stats(asv_ctx* c, uint64_t* out, void* stream)
+{
+    if (!c || !out) return fail(ASV_ERR_ARG, "null argument");
+    if (!c->march_stats) return fail(ASV_ERR_STATE, "marching-step counters not enabled");
+    HIP_TRY(hipMemcpyAsync(out, c->march_stats, kMarchStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, S(stream)));
+    HIP_TRY(hipStreamSynchronize(S(stream)));
     return ASV_OK;
 }
 

@@ -47,6 +47,7 @@ struct asv_ctx {
     unsigned long long* keys = nullptr;     // V keys for the scatter form of the sparse second pass
     int* unsafe = nullptr;                  // raised when a seed metric needs the exact clamp (JfaClip::unsafe)
     bool force_exact = false;               // asv_force_exact_clamp: raise it at every prepare
+    unsigned long long* march_stats = nullptr;   // asv_enable_march_stats: JfaClip::stats
     // page-locked host words: (seed_cap + 1) x kSumWords sums, then the
     // checksum word; every device-to-host copy of sums and checksums lands
     // here first (one explicit transfer into memory the context owns)

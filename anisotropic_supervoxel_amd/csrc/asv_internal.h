@@ -39,6 +39,9 @@ struct JfaClip {
     // with one integer maximum, exact only without NaN, so the clip's
     // columns are then evaluated tap by tap with the exact clamp.
     const int* unsafe;
+    // Optional step counters (null: none kept), summed over the launch's
+    // blocks: steps, wide steps, chunks of wide steps, per-tap steps.
+    unsigned long long* stats;
 };
 
 // Per-slot centroid sums: sum(2x+1), sum(2y+1), sum(2z+1), n, sum L, a, b, pad.
@@ -77,7 +80,8 @@ hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n);
 hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
 hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
-                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe);
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
+                      unsigned long long* stats = nullptr);
 // The first pass of an iteration when `in` holds only the seeds scattered
 // by the seed stage at `pos` (every other voxel -1): fills `out` with -1
 // and evaluates only the voxels that have a seed among their taps, at most
@@ -124,12 +128,13 @@ hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, in
 constexpr int kMarchTableIds = 16384;       // byte-table slots (dynamic LDS <= 16 KiB)
 constexpr int kMarchMaxIds = (1 << 30) - 1;  // ids below the resolved-index tag
 // ht: some clip of the launch has more than kMarchTableIds seeds (the
-// kernel whose table slots are shared and checked).
+// kernel whose table slots are shared and checked); counted: some clip has
+// step counters (JfaClip::stats; the counting kernel).
 int march_mc(int k);
 int march_max_grid(int k, int dyn, bool ht = false);
 int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn);
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht);
+                              bool ht, bool counted = false);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums);

@@ -1303,6 +1303,12 @@ template <> struct MarchMask<63> { using T = unsigned long long; };
 // A staged id after the claim: the compact index it resolved to, tagged
 // with this bit (ids of the marching pass stay below 2^30).
 constexpr int kResolved = 0x40000000;
+// Table bytes of one generation: compact indices 0 .. kIdxCap - 1, and
+// kIdxCap for every id listed past them.
+constexpr unsigned kIdxCap = 0xFDu;
+// The index byte of a cell without an id while a rebuild's list is wider
+// than one mask (any value past kIdxCap).
+constexpr unsigned kIdxNone = 0xFFu;
 
 // Slices of residue r (z = r, r + K, ...) in segment g of a clip.
 template <int K>
@@ -1343,8 +1349,13 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 // With S = 2 a step evaluates slices z and z + K from four slab masks and
 // stages the two new slabs z + 2K and z + 3K, so the barriers, claims and
 // record staging are paid once per two slices.
-// A rebuild that still leaves more than MC candidates evaluates that slice
-// per tap from global labels and records.
+// A rebuild that still leaves more than MC candidates (a wide step) keeps
+// their full compact indices in the index bytes and evaluates the step in
+// chunks of MC list entries: each chunk stages its records, builds the slab
+// masks of the indices it holds and lowers every voxel's running minimum,
+// so each candidate is still evaluated once per voxel and slice.  A list
+// past kIdxCap entries evaluates the step per tap from global labels and
+// records.
 //
 // The byte table has kMarchTableIds slots.  HT (a launch with a clip of
 // more seeds than slots): id i takes slot i mod kMarchTableIds, so a clip
@@ -1353,7 +1364,11 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 // candidate list after the claims, ids are not resolved during the claim,
 // and a mismatch makes the step evaluate per tap and the next step rebuild
 // (so no later mask holds a cell whose id was dropped).
-template <int K, int MC, int S, bool HT>
+//
+// ST: the counting form, which adds each item's steps, wide steps, their
+// chunks and per-tap steps to c.stats (when set); the kernels of ordinary
+// launches carry none of that code.
+template <int K, int MC, int S, bool HT, bool ST>
 __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
                                                    int nitems, int slot_words)
 {
@@ -1389,12 +1404,17 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     __shared__ int s_list[kListCap];
     __shared__ int s_count;
     __shared__ int s_coll;                    // the last step (gstep) that met a slot shared by two ids
+    // ST: the item's wide steps, their chunks, per-tap steps, candidate
+    // evaluations of the one-mask path, its wave rounds and wave-slices
+    __shared__ unsigned s_cnt[6];
+    __shared__ unsigned s_wmax[4];
 
     const int tid = threadIdx.x;
     for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
     if (tid == 0) {
         s_count = 0;
         s_coll = -1;
+        for (int i = 0; i < 6; ++i) s_cnt[i] = 0;
     }
     __syncthreads();
     int gstep = 0;                            // steps of this block so far (uniform)
@@ -1411,6 +1431,16 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     const int mbase = (ly + K) * R + (lx + K);
 
     int known = 0;                            // s_count as of the last barrier
+    // a wide step keeps its voxels' running keys in the table's words: the
+    // next release clears the whole table (uniform)
+    bool scrubbed = false;
+    // key words of a wide step: two per thread and slice
+    constexpr int kKeyWords = 2 * 256 * S;
+    // Wide steps (chunked lists) where their registers keep the occupancy:
+    // at K = 2 and in the hashed form they cost a wave per SIMD, and those
+    // steps evaluate per tap instead
+    constexpr bool kWide = K != 2 && !HT;
+    const bool wide_ok = kWide && slot_words >= kKeyWords;
     // a block's items increase, so its clip index only moves forward: the
     // descriptor is reloaded only when the item leaves the current clip
     int ci = -1, ci_end = 0;
@@ -1500,7 +1530,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             if (prev == old) {
                                 const int k = atomicAdd(&s_count, 1);
                                 if (k < kListCap) s_list[k] = id;
-                                byte = k < MC ? (unsigned)k : (unsigned)MC;
+                                byte = k < (int)kIdxCap ? (unsigned)k : kIdxCap;
                                 // one atomic turns the claimed byte into the
                                 // index: lanes reading the word meanwhile see
                                 // either the claim or the index
@@ -1515,20 +1545,22 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
         };
-        auto convert = [&](const int* ids, int slot) {
+        // index bytes of a staged slab: a cell without an id gets `none` (MC,
+        // the mask's bad bit, or kIdxNone in a wide step)
+        auto convert = [&](const int* ids, int slot, unsigned none) {
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
                 const int f = tid + 256 * i;
                 const int v = ids[i];
                 if (f < RA) {
-                    unsigned cix = (unsigned)MC;
+                    unsigned cix = none;
                     if (v >= kResolved) {
                         cix = (unsigned)v & 0xFFu;
                     } else if (v >= 0) {
                         const int sid = HT ? v & (kMarchTableIds - 1) : v;
                         cix = (s_slot[sid >> 2] >> ((unsigned)(sid & 3) * 8u)) & 0xFFu;
-                        if (hashed && cix < (unsigned)MC && s_list[cix] != v) {
-                            cix = (unsigned)MC;            // the slot holds another id
+                        if (hashed && cix < kIdxCap && s_list[cix] != v) {
+                            cix = none;                    // the slot holds another id
                             s_coll = gstep;
                         }
                     }
@@ -1579,6 +1611,19 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
         };
+        // a wide step's chunk: list entries [b, b + cnt) staged at 0 .. cnt - 1
+        auto stage_chunk = [&](int b, int cnt, int z) {
+            if (tid < cnt) {
+                const Staged r = load_rec(b + tid);
+                s_p[tid] = make_float4(r.a.x, r.a.y, r.a.w, r.b.x);
+                s_c[tid] = make_float4(__int_as_float(r.id), r.c.y, r.c.z, r.c.w);
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    const float dz = (float)(z + k * K) - r.a.z;
+                    s_u[k][tid] = make_float4(dz * r.b.y, dz * r.b.w, (dz * dz) * r.c.x, r.b.z);
+                }
+            }
+        };
         // index bytes are read one by one (lds_byte): at K = 1 the 16-bit
         // reads the compiler would merge them into took the pass from 335
         // to 595 us at 640x480x200
@@ -1591,14 +1636,31 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 for (int dx = -1; dx <= 1; ++dx) m |= (MaskT)1 << p[dy * K * R + dx * K];
             return m;
         };
+        // the same over the indices [b, b + cnt) of a wide step's chunk (bit
+        // index - b; the bad bit for the rest and for cells without an id)
+        auto slab_mask_chunk = [&](int slot, unsigned b, unsigned cnt) {
+            MaskT m = 0;
+            lds_byte* p = (lds_byte*)(s_idx + slot * RA + mbase);
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const unsigned i = (unsigned)p[dy * K * R + dx * K] - b;
+                    m |= (MaskT)1 << (i < cnt ? i : (unsigned)MC);
+                }
+            return m;
+        };
         // release every claimed byte and empty the list (callers barrier after)
         auto release = [&]() {
-            if (known <= kListCap) {
+            if (!scrubbed && known <= kListCap) {
                 for (int e = tid; e < known; e += 256) {
                     const int sid = HT ? s_list[e] & (kMarchTableIds - 1) : s_list[e];
                     atomicOr(&s_slot[sid >> 2], 0xFFu << ((unsigned)(sid & 3) * 8u));
                 }
             } else {
+                // after a wide step every lane has read its last key words
+                // before any are cleared (scrubbed is uniform)
+                if (scrubbed) __syncthreads();
                 for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
             }
         };
@@ -1619,6 +1681,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         bool full = true;
         for (int step = 0; step < n;) {
             bool overflow = false;
+            int nchunk = 1;                   // MC-entry chunks of the step's list (uniform)
             // without the look-ahead, this step's LAB is requested before the
             // claims and barriers, so its latency hides behind them
             uint32_t pk_cur[S];
@@ -1630,6 +1693,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             }
             if (full) {
                 release();
+                scrubbed = false;
                 __syncthreads();
                 if (tid == 0) s_count = 0;
                 int ids_all[S + 2][NRS];
@@ -1639,27 +1703,42 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 #pragma unroll
                 for (int j = 0; j < S + 2; ++j) claim(ids_all[j]);
                 __syncthreads();
-                known = s_count;
-                overflow = known > MC || exact;
+                known = __builtin_amdgcn_readfirstlane(s_count);
+                const bool wide = known > MC && wide_ok;
+                overflow = known > (wide_ok ? (int)kIdxCap : MC) || exact;
                 if (!overflow) {
+                    const int c0 = wide ? MC : known;
                     Staged rs;
-                    if (kStageEarly) rs = stage_load(0, known);
+                    if (kStageEarly) rs = stage_load(0, c0);
 #pragma unroll
-                    for (int j = 0; j < S + 2; ++j) convert(ids_all[j], j);
-                    stage(rs, 0, known, z);
+                    for (int j = 0; j < S + 2; ++j) convert(ids_all[j], j, wide ? kIdxNone : (unsigned)MC);
+                    stage(rs, 0, c0, z);
                 }
                 __syncthreads();
-                overflow = overflow || (hashed && s_coll == gstep);
+                overflow = overflow || (hashed && __builtin_amdgcn_readfirstlane(s_coll) == gstep);
                 if (!overflow) {
+                    if (wide) {
 #pragma unroll
-                    for (int j = 0; j < S + 2; ++j) m[j] = slab_mask(j);
+                        for (int j = 0; j < S + 2; ++j) m[j] = slab_mask_chunk(j, 0u, (unsigned)MC);
+                        nchunk = (known + MC - 1) / MC;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < S + 2; ++j) m[j] = slab_mask(j);
+                    }
                 }
-                full = overflow;              // rebuild again at the next step
+                full = overflow || wide;      // rebuild again at the next step
+                if (ST && c.stats && tid == 0) {
+                    if (nchunk > 1) {
+                        ++s_cnt[0];
+                        s_cnt[1] += nchunk;
+                    }
+                    if (overflow) ++s_cnt[2];
+                }
             } else {
 #pragma unroll
                 for (int k = 0; k < S; ++k) claim(ids_new[k]);
                 __syncthreads();
-                const int cnt = s_count;
+                const int cnt = __builtin_amdgcn_readfirstlane(s_count);
                 if (cnt > MC) {               // uniform: rebuild this step
                     known = cnt;
                     full = true;
@@ -1668,13 +1747,14 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 Staged rs;
                 if (kStageEarly) rs = stage_load(known, cnt);
 #pragma unroll
-                for (int k = 0; k < S; ++k) convert(ids_new[k], k);
+                for (int k = 0; k < S; ++k) convert(ids_new[k], k, (unsigned)MC);
                 stage(rs, known, cnt, z);
                 known = cnt;
                 __syncthreads();
-                if (hashed && s_coll == gstep) {
+                if (hashed && __builtin_amdgcn_readfirstlane(s_coll) == gstep) {
                     overflow = true;          // this step per tap, the next one rebuilds
                     full = true;
+                    if (ST && c.stats && tid == 0) ++s_cnt[2];
                 } else {
                     m[0] = m[S];
                     m[1] = m[S + 1];
@@ -1701,33 +1781,60 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
             }
 
-            if (mine) {
+            // one slice's candidates of the mask path, lowering `best`
+            auto eval_masks = [&](int k, float l0, float l1, float l2, unsigned long long best) {
+                MaskT mask = (m[k] | m[k + 1] | m[k + 2]) & ~kBadBit;
+                while (mask) {
+                    const int e = MC == 31 ? __builtin_ctz((uint32_t)mask) : __builtin_ctzll((unsigned long long)mask);
+                    mask &= mask - 1;
+                    const float4 p = s_p[e], u = s_u[k][e], cc = s_c[e];
+                    const float dx = fx - p.x, dy = fy - p.y;
+                    const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
+                    const float t2 = __fmaf_rn(dy, u.w, u.y);
+                    const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
+                    const float c0 = l0 - cc.y, c1 = l1 - cc.z, c2 = l2 - cc.w;
+                    const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
+                    const unsigned long long kk =
+                        ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.x);
+                    best = key_min(kk, best);
+                }
+                return best;
+            };
+            auto lab_of = [&](uint32_t pk, float& l0, float& l1, float& l2) {
+                l0 = (float)(pk & 255u);
+                l1 = (float)((pk >> 8) & 255u);
+                l2 = (float)((pk >> 16) & 255u);
+            };
+
+            if (mine && nchunk == 1) {
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     if (k >= ns) break;
                     const int zk = z + k * K;
                     const size_t v = ((size_t)zk * H + y) * W + x;
-                    const uint32_t pk = pk_here[k];
-                    const float l0 = (float)(pk & 255u), l1 = (float)((pk >> 8) & 255u),
-                                l2 = (float)((pk >> 16) & 255u);
+                    float l0, l1, l2;
+                    lab_of(pk_here[k], l0, l1, l2);
                     unsigned long long best = ASV_KEY_INIT;
                     if (!overflow) {
-                        MaskT mask = (m[k] | m[k + 1] | m[k + 2]) & ~kBadBit;
-                        while (mask) {
-                            const int e = MC == 31 ? __builtin_ctz((uint32_t)mask)
-                                                   : __builtin_ctzll((unsigned long long)mask);
-                            mask &= mask - 1;
-                            const float4 p = s_p[e], u = s_u[k][e], cc = s_c[e];
-                            const float dx = fx - p.x, dy = fy - p.y;
-                            const float t1 = __fmaf_rn(dx, p.z, __fmaf_rn(dy, p.w, u.x));
-                            const float t2 = __fmaf_rn(dy, u.w, u.y);
-                            const float q = nonneg(__fmaf_rn(dx, t1, __fmaf_rn(dy, t2, u.z)));
-                            const float c0 = l0 - cc.y, c1 = l1 - cc.z, c2 = l2 - cc.w;
-                            const float d = __fmaf_rn(ratio, q, __fmaf_rn(c2, c2, __fmaf_rn(c1, c1, c0 * c0)));
-                            const unsigned long long kk =
-                                ((unsigned long long)__float_as_uint(d) << 32) | __float_as_uint(cc.x);
-                            best = key_min(kk, best);
+                        if (ST && c.stats) {
+                            // this lane's candidates, the wave's rounds (its
+                            // busiest lane) and one wave-slice
+                            const MaskT mk = (m[k] | m[k + 1] | m[k + 2]) & ~kBadBit;
+                            const unsigned pc = MC == 31 ? __builtin_popcount((uint32_t)mk)
+                                                         : __builtin_popcountll((unsigned long long)mk);
+                            const unsigned long long act = __ballot(1);
+                            const bool leader = lane == __builtin_ctzll(act);
+                            // every active lane resets the slot, so the reset
+                            // is one store ahead of the one atomic for all
+                            s_wmax[wave] = 0u;
+                            atomicMax(&s_wmax[wave], pc);
+                            atomicAdd(&s_cnt[3], pc);
+                            if (leader) {
+                                atomicAdd(&s_cnt[4], s_wmax[wave]);
+                                atomicAdd(&s_cnt[5], 1u);
+                            }
                         }
+                        best = eval_masks(k, l0, l1, l2, best);
                     } else {
                         // per tap from global labels and records, the tap
                         // coordinates formed as the loop goes (fewer live
@@ -1759,9 +1866,56 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
                 }
             }
+            if (kWide && nchunk > 1) {
+                // a wide step: each slice's best key lowered chunk by chunk,
+                // kept between chunks in the table's words (this thread's
+                // own: word k 512 + tid low, k 512 + 256 + tid high)
+                for (int ch = 0;;) {
+                    const bool last = ch + 1 == nchunk;
+                    if (mine) {
+#pragma unroll
+                        for (int k = 0; k < S; ++k) {
+                            if (k >= ns) break;
+                            float l0, l1, l2;
+                            lab_of(pk_here[k], l0, l1, l2);
+                            uint32_t* kw = s_slot + k * 512 + tid;
+                            unsigned long long best = ASV_KEY_INIT;
+                            if (ch > 0) best = ((unsigned long long)kw[256] << 32) | kw[0];
+                            best = eval_masks(k, l0, l1, l2, best);
+                            if (!last) {
+                                kw[0] = (uint32_t)best;
+                                kw[256] = (uint32_t)(best >> 32);
+                            } else {
+                                const size_t v = ((size_t)(z + k * K) * H + y) * W + x;
+                                g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
+                            }
+                        }
+                    }
+                    if (last) break;
+                    ++ch;
+                    // every lane is done with this chunk's records before
+                    // the next chunk's replace them
+                    const int b = ch * MC, cnt = min(MC, known - b);
+                    __syncthreads();
+                    stage_chunk(b, cnt, z);
+                    __syncthreads();
+#pragma unroll
+                    for (int j = 0; j < S + 2; ++j) m[j] = slab_mask_chunk(j, (unsigned)b, (unsigned)cnt);
+                }
+                scrubbed = true;
+            }
             z += S * K;
             step += S;
             ++gstep;
+        }
+        if (ST && c.stats) __syncthreads();   // every wave's counts of the item are in (uniform)
+        if (ST && c.stats && tid == 0) {
+            unsigned long long* row = c.stats + (K - 1) * 8;   // this step size's row
+            atomicAdd(row, (unsigned long long)((n + S - 1) / S));
+            for (int i = 0; i < 6; ++i) {
+                atomicAdd(row + 1 + i, (unsigned long long)s_cnt[i]);
+                s_cnt[i] = 0;
+            }
         }
     }
 }
@@ -2066,42 +2220,46 @@ int march_s(int k)
 
 using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
 
-template <int K, bool HT>
+template <int K, bool HT, bool ST>
 static MarchFn march_fn(int mc, int sl)
 {
-    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2, HT> : k_jfa_march<K, 63, 2, HT>;
-    return mc == 31 ? k_jfa_march<K, 31, 1, HT> : k_jfa_march<K, 63, 1, HT>;
+    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2, HT, ST> : k_jfa_march<K, 63, 2, HT, ST>;
+    return mc == 31 ? k_jfa_march<K, 31, 1, HT, ST> : k_jfa_march<K, 63, 1, HT, ST>;
 }
 
-// The marching kernel for step k, candidate capacity mc, slices per step sl;
-// ht: some clip of the launch has more seeds than the table has slots.
-template <bool HT>
-static MarchFn march_kernel_ht(int k, int mc, int sl)
+template <bool HT, bool ST>
+static MarchFn march_kernel_k(int k, int mc, int sl)
 {
     switch (k) {
-    case 1: return march_fn<1, HT>(mc, sl);   case 2: return march_fn<2, HT>(mc, sl);
-    case 3: return march_fn<3, HT>(mc, sl);   case 4: return march_fn<4, HT>(mc, sl);
-    case 5: return march_fn<5, HT>(mc, sl);   case 6: return march_fn<6, HT>(mc, sl);
-    case 7: return march_fn<7, HT>(mc, sl);   case 8: return march_fn<8, HT>(mc, sl);
-    case 9: return march_fn<9, HT>(mc, sl);   case 10: return march_fn<10, HT>(mc, sl);
-    case 11: return march_fn<11, HT>(mc, sl); case 12: return march_fn<12, HT>(mc, sl);
-    case 13: return march_fn<13, HT>(mc, sl); case 14: return march_fn<14, HT>(mc, sl);
-    case 15: return march_fn<15, HT>(mc, sl); case 16: return march_fn<16, HT>(mc, sl);
+    case 1: return march_fn<1, HT, ST>(mc, sl);   case 2: return march_fn<2, HT, ST>(mc, sl);
+    case 3: return march_fn<3, HT, ST>(mc, sl);   case 4: return march_fn<4, HT, ST>(mc, sl);
+    case 5: return march_fn<5, HT, ST>(mc, sl);   case 6: return march_fn<6, HT, ST>(mc, sl);
+    case 7: return march_fn<7, HT, ST>(mc, sl);   case 8: return march_fn<8, HT, ST>(mc, sl);
+    case 9: return march_fn<9, HT, ST>(mc, sl);   case 10: return march_fn<10, HT, ST>(mc, sl);
+    case 11: return march_fn<11, HT, ST>(mc, sl); case 12: return march_fn<12, HT, ST>(mc, sl);
+    case 13: return march_fn<13, HT, ST>(mc, sl); case 14: return march_fn<14, HT, ST>(mc, sl);
+    case 15: return march_fn<15, HT, ST>(mc, sl); case 16: return march_fn<16, HT, ST>(mc, sl);
     default: return nullptr;
     }
 }
 
-static MarchFn march_kernel(int k, int mc, int sl, bool ht = false)
+// The marching kernel for step k, candidate capacity mc, slices per step sl;
+// ht: some clip of the launch has more seeds than the table has slots;
+// counted: the counting form (built for the direct table only: a hashed
+// launch runs uncounted).
+static MarchFn march_kernel(int k, int mc, int sl, bool ht = false, bool counted = false)
 {
-    return ht ? march_kernel_ht<true>(k, mc, sl) : march_kernel_ht<false>(k, mc, sl);
+    if (ht) return march_kernel_k<true, false>(k, mc, sl);
+    return counted ? march_kernel_k<false, true>(k, mc, sl) : march_kernel_k<false, false>(k, mc, sl);
 }
 
 // Dynamic LDS of the marching pass: the byte-per-id table for ids < amax,
-// in whole KiB.
+// in whole KiB, and at least the 4 KiB a wide step keeps its keys in.
 static int march_dyn_bytes(int amax)
 {
     const int bytes = ((amax > 0 ? amax : 1) + 3) / 4 * 4;
-    return (bytes + 1023) / 1024 * 1024;
+    const int kib = (bytes + 1023) / 1024 * 1024;
+    return kib > 4096 ? kib : 4096;
 }
 
 // Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
@@ -2197,9 +2355,9 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
 }
 
 hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
-                        int dyn, bool ht)
+                        int dyn, bool ht, bool counted)
 {
-    MarchFn fn = march_kernel(k, march_mc(k), march_s(k), ht);
+    MarchFn fn = march_kernel(k, march_mc(k), march_s(k), ht, counted);
     if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
     JfaClip a0 = one;
     const JfaClip* a1 = dev;
@@ -2209,22 +2367,22 @@ hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip
 }
 
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht)
+                              bool ht, bool counted)
 {
     JfaClip none{};
-    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht);
+    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht, counted);
 }
 
 template <int K>
 static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                              const SeedRec* rec, float ratio, const int* unsafe)
+                              const SeedRec* rec, float ratio, const int* unsafe, unsigned long long* stats)
 {
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     if (d.A <= kMarchMaxIds) {
-        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe};
+        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe, stats};
         int nitems = 0, dyn = 0;
         const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
-        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds);
+        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds, stats != nullptr);
     }
     int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
@@ -2282,27 +2440,28 @@ hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n)
 }
 
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
-                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe)
+                      const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
+                      unsigned long long* stats)
 {
     int64_t ntiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile) * d.D;
     if (tie_rule == 0 && ntiles < (1ll << 31)) {
         switch (k) {
-        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio, unsafe);
-        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio, unsafe);
+        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio, unsafe, stats);
         default: break;
         }
     }
