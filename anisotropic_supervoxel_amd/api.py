@@ -290,10 +290,14 @@ class Clip:
         wide steps (candidate lists evaluated in chunks), their chunks and
         per-tap steps; for the steps within one candidate mask, candidate
         evaluations, wave rounds (each 64-voxel wave-slice's busiest voxel)
-        and wave-slices; by_step adds the same per JFA step size."""
-        out = np.zeros((16, 8), np.uint64)
+        and wave-slices; for the steps that extend a table generation, the
+        waves' cycles per phase (claims, conversion, masks, load issue,
+        evaluation) and those wave-steps; by_step adds the same per JFA step
+        size."""
+        out = np.zeros((16, 16), np.uint64)
         check(self.lib.asv_download_march_stats(self.h, ptr(out), self.stream))
-        names = ("steps", "wide", "chunks", "per_tap", "candidates", "wave_rounds", "wave_slices")
+        names = ("steps", "wide", "chunks", "per_tap", "candidates", "wave_rounds", "wave_slices",
+                 "cyc_claim", "cyc_convert", "cyc_mask", "cyc_prefetch", "cyc_eval", "timed_wave_steps")
 
         def row(r):
             return {n: int(r[i]) for i, n in enumerate(names)}

@@ -1095,9 +1095,9 @@ int asv_force_exact_clamp(asv_ctx* c, int32_t on)
     return ASV_OK;
 }
 
-// Marching-step counters (asv_enable_march_stats): a row of 8 words (7
+// Marching-step counters (asv_enable_march_stats): a row of 16 words (13
 // used) per step size 1..16.
-constexpr int kMarchStatWords = 16 * 8;
+constexpr int kMarchStatWords = 16 * 16;
 
 int asv_enable_march_stats(asv_ctx* c, int32_t on)
 {

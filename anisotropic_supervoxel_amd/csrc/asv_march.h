@@ -130,13 +130,20 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // evaluations of the one-mask path, its wave rounds and wave-slices
     __shared__ unsigned s_cnt[6];
     __shared__ unsigned s_wmax[4];
+    // ST: cycles the waves spent in the phases of steps that extend a table
+    // generation (claims, conversion and staging, masks, prefetch issue,
+    // evaluation), and those wave-steps
+    __shared__ unsigned long long s_tcyc[6];
 
     const int tid = threadIdx.x;
     for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
     if (tid == 0) {
         s_count = 0;
         s_coll = -1;
-        for (int i = 0; i < 6; ++i) s_cnt[i] = 0;
+        for (int i = 0; i < 6; ++i) {
+            s_cnt[i] = 0;
+            s_tcyc[i] = 0;
+        }
     }
     __syncthreads();
     int gstep = 0;                            // steps of this block so far (uniform)
@@ -401,8 +408,19 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             if (kLabAhead && mine && k < n) pk_next[k] = g_lab[((size_t)(z + k * K) * H + y) * W + x];
         }
         bool full = true;
+        // ST: the phase clock of this wave (uniform)
+        unsigned long long t_mark = 0;
+        bool timed = false;
+        auto tick = [&](int ph) {
+            if (ST && c.stats && timed) {
+                const unsigned long long t = __builtin_amdgcn_s_memtime();
+                if (ph >= 0 && lane == 0) atomicAdd(&s_tcyc[ph], t - t_mark);
+                t_mark = t;
+            }
+        };
         for (int step = 0; step < n;) {
             bool overflow = false;
+            timed = ST && !full;
             int nchunk = 1;                   // MC-entry chunks of the step's list (uniform)
             // without the look-ahead, this step's LAB is requested before the
             // claims and barriers, so its latency hides behind them
@@ -457,9 +475,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                     if (overflow) ++s_cnt[2];
                 }
             } else {
+                tick(-1);
 #pragma unroll
                 for (int k = 0; k < S; ++k) claim(ids_new[k]);
                 __syncthreads();
+                tick(0);
                 const int cnt = __builtin_amdgcn_readfirstlane(s_count);
                 if (cnt > MC) {               // uniform: rebuild this step
                     known = cnt;
@@ -473,6 +493,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 stage(rs, known, cnt, z);
                 known = cnt;
                 __syncthreads();
+                tick(1);
                 if (hashed && __builtin_amdgcn_readfirstlane(s_coll) == gstep) {
                     overflow = true;          // this step per tap, the next one rebuilds
                     full = true;
@@ -483,6 +504,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 #pragma unroll
                     for (int k = 0; k < S; ++k) m[2 + k] = slab_mask(k);
                 }
+                tick(2);
             }
             const int ns = min(S, n - step);  // slices of this step
             // the next step's new slabs (and LAB), loaded while this step is evaluated
@@ -528,6 +550,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 l2 = (float)((pk >> 16) & 255u);
             };
 
+            tick(3);
             if (mine && nchunk == 1) {
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
@@ -626,17 +649,21 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 }
                 scrubbed = true;
             }
+            tick(4);
+            if (ST && c.stats && timed && lane == 0) atomicAdd(&s_tcyc[5], 1ull);
             z += S * K;
             step += S;
             ++gstep;
         }
         if (ST && c.stats) __syncthreads();   // every wave's counts of the item are in (uniform)
         if (ST && c.stats && tid == 0) {
-            unsigned long long* row = c.stats + (K - 1) * 8;   // this step size's row
+            unsigned long long* row = c.stats + (K - 1) * 16;   // this step size's row
             atomicAdd(row, (unsigned long long)((n + S - 1) / S));
             for (int i = 0; i < 6; ++i) {
                 atomicAdd(row + 1 + i, (unsigned long long)s_cnt[i]);
                 s_cnt[i] = 0;
+                atomicAdd(row + 7 + i, s_tcyc[i]);
+                s_tcyc[i] = 0;
             }
         }
     }

@@ -235,15 +235,19 @@ int asv_host_free(void* p);
 int asv_force_exact_clamp(asv_ctx* ctx, int32_t on);
 /* Counters of the marching full passes, kept while enabled (on = 1 zeroes
  * them; on = 0 drops them), summed over every later run of the context.
- * out holds 16 rows of 8 words, row k - 1 for JFA step k: [0] steps (one
+ * out holds 16 rows of 16 words, row k - 1 for JFA step k: [0] steps (one
  * step: two slices of one 16 x 16 column of voxels), [1] wide steps (more
  * seeds in reach than one candidate mask holds, evaluated in chunks), [2]
  * chunks of the wide steps, [3] steps evaluated per tap; of the steps
  * within one mask: [4] candidate evaluations (voxel-slices x candidates),
  * [5] wave rounds (per 64-voxel wave and slice, its busiest voxel's
- * candidates), [6] wave-slices; [7] 0.  Runs with counters enabled use a
- * counting form of the marching kernel.  Tests and tools only; no
- * counterpart in the reference. */
+ * candidates), [6] wave-slices; of the steps that extend a table
+ * generation, shader-clock cycles the waves spent in [7] claims, [8] index
+ * conversion and record staging, [9] masks, [10] next-step load issue,
+ * [11] evaluation and stores, and [12] those wave-steps; [13..15] 0.  Runs
+ * with counters enabled use a counting form of the marching kernel (the
+ * cycle counts are of that form).  Tests and tools only; no counterpart in
+ * the reference. */
 int asv_enable_march_stats(asv_ctx* ctx, int32_t on);
 int asv_download_march_stats(asv_ctx* ctx, uint64_t* out, void* stream);
 /* Fault injection for tests of failure handling: from now on every device

@@ -71,7 +71,10 @@ def test_counting_kernel_matches_plain(have_gpu):
         c.enable_march_stats()
         zero = c.march_stats()
     assert np.array_equal(plain, counted)
-    assert one["steps"] > 0 and two == {k: 2 * v for k, v in one.items()}
+    # the counts double over a second identical run (cycle counts are timings)
+    counts = [k for k in one if not k.startswith("cyc_")]
+    assert one["steps"] > 0 and all(two[k] == 2 * one[k] for k in counts)
+    assert one["cyc_eval"] > 0 and one["timed_wave_steps"] > 0
     assert all(v == 0 for v in zero.values())
     # a wave's rounds cover each of its voxels' candidates
     assert one["candidates"] <= 64 * one["wave_rounds"] and one["wave_slices"] > 0
