@@ -61,6 +61,7 @@ struct PassGroup {
     bool split = false;            // tuned: the clips' own launches are faster
     bool ht = false;               // a clip has more seeds than the marching table has slots
     bool counted = false;          // a clip keeps marching-step counters
+    bool dense = false;            // few voxels per seed over the clips (march_dense)
 };
 
 JfaClip descriptor(const asv_ctx* c, int p)
@@ -121,6 +122,7 @@ void build_groups(const std::vector<asv_ctx*>& clips, std::vector<PassGroup>* gr
             }
         }
         for (PassGroup& g : row) {
+            if (g.batched) g.dense = march_dense(g.desc.data(), (int)g.desc.size());
             g.dev_off = *n_desc;
             *n_desc += g.desc.size();
             groups->push_back(std::move(g));
@@ -134,7 +136,7 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
 {
     if (g.batched && !g.split) {
         BHIP(launch_march_batch(st, g.K, dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn, g.ht,
-                                g.counted));
+                                g.counted, g.dense));
         return ASV_OK;
     }
     for (int i : g.clips) {
@@ -244,7 +246,7 @@ int autotune(asv_batch* b)
     float total = 0.0f;
     for (PassGroup& g : b->groups) {
         if (g.batched) {
-            const int gmax = march_max_grid(g.K, g.dyn, g.ht);
+            const int gmax = march_max_grid(g.K, g.dyn, g.ht, g.dense);
             struct Shape {
                 int gl, pb, cap;
                 bool split;
@@ -563,7 +565,7 @@ int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, v
                     if (!rc) rc = to_batch(i);
                 if (!rc) {
                     const hipError_t e = launch_march_batch(bst, g.K, sl.dev + g.dev_off, (int)g.desc.size(), g.grid,
-                                                            g.nitems, g.dyn, g.ht, g.counted);
+                                                            g.nitems, g.dyn, g.ht, g.counted, g.dense);
                     if (e != hipSuccess) rc = bfail(ASV_ERR_HIP, "marching launch", hipGetErrorString(e));
                     batch_ahead = true;
                 }

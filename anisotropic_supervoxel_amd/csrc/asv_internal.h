@@ -130,11 +130,14 @@ constexpr int kMarchMaxIds = (1 << 30) - 1;  // ids below the resolved-index tag
 // ht: some clip of the launch has more than kMarchTableIds seeds (the
 // kernel whose table slots are shared and checked); counted: some clip has
 // step counters (JfaClip::stats; the counting kernel).
-int march_mc(int k);
-int march_max_grid(int k, int dyn, bool ht = false);
+// dense: the launch's clips hold few voxels per seed (march_dense), which
+// selects the wider candidate masks (march_mc).
+bool march_dense(const JfaClip* c, int n);
+int march_mc(int k, bool dense = false);
+int march_max_grid(int k, int dyn, bool ht = false, bool dense = false);
 int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn);
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht, bool counted = false);
+                              bool ht, bool counted = false, bool dense = false);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums);

@@ -1473,11 +1473,27 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
     return hipGetLastError();
 }
 
+// A launch whose clips hold fewer than this many voxels per seed (grid
+// step below about 11) is dense: a column segment meets more seeds than 31
+// candidates hold, and the 64-bit masks avoid the rebuilds and per-tap steps.
+constexpr int64_t kDenseVoxelsPerSeed = 1500;
+
+bool march_dense(const JfaClip* c, int n)
+{
+    int64_t v = 0, a = 0;
+    for (int i = 0; i < n; ++i) {
+        v += (int64_t)c[i].W * c[i].H * c[i].D;
+        a += c[i].A;
+    }
+    return a > 0 && v < kDenseVoxelsPerSeed * a;
+}
+
 // Candidate capacity of the marching pass at step k: 32-bit masks (31
 // candidates per table generation) or 64-bit masks (63).  Small steps meet
-// few distinct seeds along a column; the environment variable ASV_MARCH_MC
-// (31 or 63) overrides the choice for every step.
-int march_mc(int k)
+// few distinct seeds along a column unless the seeds are dense; the
+// environment variable ASV_MARCH_MC (31 or 63) overrides the choice for
+// every step.
+int march_mc(int k, bool dense)
 {
     static const int forced = [] {          // read once, thread-safe initialisation
         const char* e = getenv("ASV_MARCH_MC");
@@ -1486,8 +1502,10 @@ int march_mc(int k)
     }();
     if (forced) return forced;
     // measured at 640x480x200 (31 vs 63): K = 1 313 vs 354 us, K = 2 332 vs
-    // 371, K = 3 480 vs 513, K = 5 430 vs 425 (two runs each, within 0.3 %)
-    return k <= 4 ? 31 : 63;
+    // 371, K = 3 480 vs 513, K = 5 430 vs 425 (two runs each, within 0.3 %);
+    // at 640x480x10 with 12288 seeds (250 voxels per seed): K = 1 43.8 vs
+    // 30.5 us, K = 2 132.6 vs 51.5, K = 3 38.2 vs 37.9
+    return k <= 4 && !dense ? 31 : 63;
 }
 
 // Slices of a chain evaluated per step of the marching pass at step k: two
@@ -1510,7 +1528,7 @@ int march_s(int k)
 
 // The marching kernel for step k, candidate capacity mc, slices per step sl;
 // ht: some clip of the launch has more seeds than the table has slots;
-// counted: the counting form (direct table only).
+// counted: the counting form.
 static MarchFn march_kernel(int k, int mc, int sl, bool ht = false, bool counted = false)
 {
     switch ((k - 1) >> 2) {
@@ -1533,11 +1551,11 @@ static int march_dyn_bytes(int amax)
 
 // Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
 // 8), cached per step, capacity and table size.
-int march_max_grid(int k, int dyn, bool ht)
+int march_max_grid(int k, int dyn, bool ht, bool dense)
 {
     static int cache[17][2][2][2][kMarchTableIds / 1024 + 1];
     if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchTableIds) return 0;
-    const int mc = march_mc(k), sl = march_s(k);
+    const int mc = march_mc(k, dense), sl = march_s(k);
     int& slot = cache[k][mc == 63][sl == 2][ht][dyn / 1024];
     int g = __atomic_load_n(&slot, __ATOMIC_RELAXED);
     if (g == 0) {
@@ -1589,7 +1607,7 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
     for (int i = 0; i < n; ++i) amax = c[i].A > amax ? c[i].A : amax;
     if (amax > kMarchMaxIds) return 0;
     const int lds = march_dyn_bytes(amax < kMarchTableIds ? amax : kMarchTableIds);
-    int gmax = march_max_grid(k, lds, amax > kMarchTableIds);
+    int gmax = march_max_grid(k, lds, amax > kMarchTableIds, march_dense(c, n));
     if (grid_limit > 0 && grid_limit < gmax) gmax = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
     const int64_t pmax = gmax / 8;                       // resident blocks per XCD
     int64_t columns = 0;
@@ -1624,9 +1642,9 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
 }
 
 hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
-                        int dyn, bool ht, bool counted)
+                        int dyn, bool ht, bool counted, bool dense)
 {
-    MarchFn fn = march_kernel(k, march_mc(k), march_s(k), ht, counted);
+    MarchFn fn = march_kernel(k, march_mc(k, dense), march_s(k), ht, counted);
     if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
     JfaClip a0 = one;
     const JfaClip* a1 = dev;
@@ -1636,10 +1654,10 @@ hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip
 }
 
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht, bool counted)
+                              bool ht, bool counted, bool dense)
 {
     JfaClip none{};
-    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht, counted);
+    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht, counted, dense);
 }
 
 template <int K>
@@ -1651,7 +1669,9 @@ static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, 
         JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe, stats};
         int nitems = 0, dyn = 0;
         const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
-        if (g > 0) return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds, stats != nullptr);
+        if (g > 0)
+            return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds, stats != nullptr,
+                                march_dense(&c, 1));
     }
     int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,

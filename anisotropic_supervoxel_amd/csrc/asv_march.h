@@ -166,9 +166,9 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // key words of a wide step: two per thread and slice
     constexpr int kKeyWords = 2 * 256 * S;
     // Wide steps (chunked lists) where their registers keep the occupancy:
-    // at K = 2 and in the hashed form they cost a wave per SIMD, and those
-    // steps evaluate per tap instead
-    constexpr bool kWide = K != 2 && !HT;
+    // at K = 2, and in the hashed form above K = 3, they cost a wave per
+    // SIMD, and those steps evaluate per tap instead
+    constexpr bool kWide = K != 2 && (!HT || K <= 3);
     const bool wide_ok = kWide && slot_words >= kKeyWords;
     // a block's items increase, so its clip index only moves forward: the
     // descriptor is reloaded only when the item leaves the current clip
@@ -680,8 +680,7 @@ static MarchFn march_fn(int mc, int sl)
 
 // Instances of steps 4P+1 .. 4P+4 (defined by asv_march.hip built with
 // ASV_MARCH_PART = P): the kernel for step k, capacity mc, slices per step
-// sl; ht selects the hashed-table form, counted the counting form (built for
-// the direct table only: a hashed launch runs uncounted).
+// sl; ht selects the hashed-table form, counted the counting form.
 template <int P>
 MarchFn march_kernel_part(int k, int mc, int sl, bool ht, bool counted);
 

@@ -14,7 +14,7 @@ namespace {
 template <int K>
 MarchFn pick(int mc, int sl, bool ht, bool counted)
 {
-    if (ht) return march_fn<K, true, false>(mc, sl);
+    if (ht) return counted ? march_fn<K, true, true>(mc, sl) : march_fn<K, true, false>(mc, sl);
     return counted ? march_fn<K, false, true>(mc, sl) : march_fn<K, false, false>(mc, sl);
 }
 
