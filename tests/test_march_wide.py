@@ -1,9 +1,10 @@
 """Wide marching steps: a step whose blocks' columns reach more seeds than
-one candidate mask holds (31 at steps 1-4, 63 from step 5) evaluates its
-list in chunks of that many entries; a list past 253 entries, or any
-overflowing step at step 2, evaluates per tap.  The marching-step counters
-show which of these paths each case took; labels, sums and every
-iteration's seed records are bit-exact against the oracle either way."""
+one candidate mask holds (31 at steps 1-4 of a launch with at least 1500
+voxels per seed, 63 otherwise) evaluates its list in chunks of that many
+entries; a list past 253 entries, or any overflowing step at step 2,
+evaluates per tap.  The marching-step counters show which of these paths
+each case took; labels, sums and every iteration's seed records are
+bit-exact against the oracle either way."""
 import numpy as np
 import pytest
 
@@ -22,7 +23,7 @@ def _run_counted(rgb, fwd, bwd, K, iters):
         c.upload(rgb, fwd, bwd)
         c.run()
         got = dict(labels=c.labels(), sums=c.sums(), A=c.A, trace=c.trace())
-        stats = c.march_stats()
+        stats = c.march_stats(by_step=True)
     return got, stats
 
 
@@ -35,21 +36,34 @@ def _assert_parity(got, rgb, fwd, bwd, K, iters):
     assert np.array_equal(got["sums"][:, :7], want_sums)
 
 
-@pytest.mark.parametrize("seed,shape,K,iters", [
-    # 14 voxels per seed, marching steps 1, 2, 1: step-1 lists of several chunks
-    (8, (12, 48, 48), 2000, 3),
-    # grid step 10, marching steps 3, 2, 1, 2, 1
-    (30, (60, 160, 200), 1900, 3),
-    # 4 voxels per seed: step-1 lists past 253 entries (per tap)
-    (31, (8, 32, 32), 2000, 2),
+@pytest.mark.parametrize("seed,shape,K,iters,expect", [
+    # 14 voxels per seed (dense: 63-entry masks), marching steps 1, 2, 1:
+    # step-1 lists of several 63-entry chunks, step-2 overflows per tap
+    (8, (12, 48, 48), 2000, 3, {(1, "wide"), (2, "per_tap")}),
+    # 1778 voxels per seed (31-entry masks at steps 1-4), marching steps 3, 2,
+    # 1, 2, 1: wide step-3 lists of two 31-entry chunks
+    (44, (60, 160, 200), 1100, 3, {(3, "wide")}),
+    # 1016 voxels per seed (dense), grid step 10: every list in one mask
+    (30, (60, 160, 200), 1900, 3, set()),
+    # 4 voxels per seed: step-1 lists past 253 entries (per tap) and wide ones
+    (31, (8, 32, 32), 2000, 2, {(1, "wide"), (1, "per_tap"), (2, "per_tap")}),
+    # 19890 seeds, 16 voxels per seed: the hashed table's wide steps at step 1
+    (47, (16, 160, 128), 20000, 2, {(1, "wide")}),
 ])
-def test_wide_steps_bit_exact(have_gpu, seed, shape, K, iters):
+def test_wide_steps_bit_exact(have_gpu, seed, shape, K, iters, expect):
     D, H, W = shape
     rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 6, 2.0, 4.0)
     got, st = _run_counted(rgb, fwd, bwd, K, iters)
-    print(f"case {seed} {shape} K={K}: {st}", flush=True)
+    by = st["by_step"]
+    print(f"case {seed} {shape} K={K}: " + "; ".join(
+        f"step {k}: wide {r['wide']} chunks {r['chunks']} per_tap {r['per_tap']} of {r['steps']}"
+        for k, r in by.items()), flush=True)
     assert st["steps"] > 0
     assert st["chunks"] >= 2 * st["wide"]
+    for k, path in expect:
+        assert by[k][path] > 0, (k, path)
+    if not expect:
+        assert st["wide"] == 0 and st["per_tap"] == 0
     _assert_parity(got, rgb, fwd, bwd, K, iters)
 
 
