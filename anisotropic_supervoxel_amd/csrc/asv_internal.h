@@ -102,16 +102,17 @@ hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n)
 
 // How pass i of an iteration that began with the seed stage runs, for a
 // schedule `steps`, A seeds and V voxels: 1 the first pass over the
-// scattered seeds alone, 2 the second pass over the first's output when it
-// is sparse (about 729 voxels per seed against a third of the volume), 0
-// the full marching/tiled pass.
+// scattered seeds alone, when their neighbourhoods (27 voxels per seed at
+// step 1) are at most a sixteenth of the volume (denser seeds make the full
+// pass cheaper: 10-frame windows at 250 voxels per seed, 640x480 31 + 6 us
+// fill against 23 us, 1280x720 78 + 7 against 67); 2 the second pass over
+// the first's output when that is sparse (about 729 voxels per seed against
+// a third of the volume); 0 the full marching/tiled pass.
 inline int seed_pass_kind(const std::vector<int>& steps, int64_t A, int64_t V, size_t i)
 {
-    if (steps.empty()) return 0;
-    if (i == 0 && steps[0] <= kSeedPassMaxStep) return 1;
-    if (i == 1 && steps.size() > 1 && steps[0] <= kSeedPassMaxStep && steps[1] <= kSeedPassMaxStep &&
-        A * 729 * 3 <= V)
-        return 2;
+    if (steps.empty() || steps[0] > kSeedPassMaxStep || A * 27 * 16 > V) return 0;
+    if (i == 0) return 1;
+    if (i == 1 && steps.size() > 1 && steps[1] <= kSeedPassMaxStep && A * 729 * 3 <= V) return 2;
     return 0;
 }
 hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
