@@ -40,6 +40,7 @@ SIGNATURES = {
     "asv_prepare": (ctypes.c_int, [_vp, _vp]),
     "asv_seed_stage": (ctypes.c_int, [_vp, _i32, _vp]),
     "asv_flood": (ctypes.c_int, [_vp, _i32, _vp]),
+    "asv_flood_passes": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "asv_centroids": (ctypes.c_int, [_vp, _i32, _vp]),
     "asv_run": (ctypes.c_int, [_vp, _vp]),
     "asv_run_loop": (ctypes.c_int, [_vp, _vp]),

@@ -248,6 +248,10 @@ class Clip:
     def flood(self, it):
         check(self.lib.asv_flood(self.h, it, self.stream))
 
+    def flood_passes(self, it, npasses):
+        """The first npasses passes of iteration it only (tests and tools)."""
+        check(self.lib.asv_flood_passes(self.h, it, npasses, self.stream))
+
     def centroids(self, it):
         check(self.lib.asv_centroids(self.h, it, self.stream))
 

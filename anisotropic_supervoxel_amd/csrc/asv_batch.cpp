@@ -153,6 +153,12 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
             BHIP(launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], g.K, ratio, c->p.tie_rule,
                                    c->keys));
             break;
+        case 3:                                     // both first passes, into the buffer pass 2 reads
+            BHIP(launch_seed_bricks(st, c->d, c->pos, c->rec, c->lab, c->labels[0], c->steps[0], c->steps[1], ratio,
+                                    c->mult[1] * c->ratio, c->bricks));
+            break;
+        case 4:                                     // done by pass 0
+            break;
         default:
             BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe, c->march_stats));
             break;

@@ -174,21 +174,36 @@ void free_ctx(asv_ctx* c)
                     c->trace, c->map_table, c->post_scratch, c->ck_word, c->keys, c->unsafe, c->march_stats};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    free_seed_bricks(&c->bricks);
     if (c->sums_host) (void)hipHostFree(c->sums_host);
 }
 
+// Whether a context keeps the brick lists of the fused first two passes
+// (launch_seed_bricks): the lowest-id rule, a schedule of two passes or
+// more starting with a step of at most kSeedFuseMaxK1, coordinates that fit
+// 16 bits, and A seeds sparse enough for the second pass over the seeds'
+// neighbourhoods.  Whether a run takes them is decided per run
+// (asv_ctx::fused_seed_passes).
+bool wants_bricks(const asv_params* p, const std::vector<int>& steps, int64_t A)
+{
+    const int64_t V = (int64_t)p->width * p->height * p->depth;
+    return p->tie_rule == ASV_TIE_LOWEST_ID && steps.size() > 1 && steps[0] <= kSeedFuseMaxK1 &&
+           seed_pass_kind(steps, A, V, 1) == 2 && p->width < 32768 && p->height < 32768 && p->depth < 32768;
+}
+
 // Whether a context keeps the keys of the scatter-form second pass: the
-// lowest-id rule, a sparse second pass, coordinates that fit 16 bits.
+// lowest-id rule, a sparse second pass that the fused form does not take
+// (a first step above kSeedFuseMaxK1), coordinates that fit 16 bits.
 bool wants_keys(const asv_params* p, const std::vector<int>& steps, int64_t A)
 {
     const int64_t V = (int64_t)p->width * p->height * p->depth;
-    return p->tie_rule == ASV_TIE_LOWEST_ID && seed_pass_kind(steps, A, V, 1) == 2 && p->width < 32768 &&
-           p->height < 32768 && p->depth < 32768;
+    return p->tie_rule == ASV_TIE_LOWEST_ID && seed_pass_kind(steps, A, V, 1) == 2 && !wants_bricks(p, steps, A) &&
+           p->width < 32768 && p->height < 32768 && p->depth < 32768;
 }
 
-int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace, bool keys)
+int64_t footprint_of(int64_t V, int64_t A, int iters, bool trace, bool keys, int64_t bricks)
 {
-    int64_t b = 0;
+    int64_t b = bricks;
     if (keys) b += V * 8;        // scatter keys of the sparse second pass
     b += V * 3;                  // rgb
     b += V * 16;                 // fwd + bwd
@@ -247,7 +262,8 @@ int64_t asv_footprint(const asv_params* p)
     std::vector<int> st;
     std::vector<float> mu;
     schedule_list(p->schedule, s < 1 ? 1 : s, p->width, st, mu);
-    return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false, wants_keys(p, st, A));
+    return footprint_of((int64_t)p->width * p->height * p->depth, A, p->lloyd_iters, false, wants_keys(p, st, A),
+                        wants_bricks(p, st, A) ? seed_brick_bytes(p->width, p->height, p->depth, A, st[0] + st[1]) : 0);
 }
 
 int asv_plan(const asv_params* p, int32_t* seeds_placed, int32_t* grid_step, int32_t* steps, float* mult,
@@ -321,6 +337,8 @@ int create_ctx(const asv_params* p, std::vector<int>&& xyz, int s, asv_ctx** out
         e = dalloc(&c->keys, (size_t)V, &tb);
         if (e == hipSuccess) e = launch_fill_keys(nullptr, c->keys, V);
     }
+    if (e == hipSuccess && wants_bricks(p, c->steps, A))
+        e = alloc_seed_bricks(&c->bricks, p->width, p->height, p->depth, Acap, c->steps[0] + c->steps[1], &tb);
     if (e != hipSuccess) {
         free_ctx(c);
         delete c;
@@ -486,6 +504,8 @@ bool asv::ctx_reusable(const asv_ctx* c, const asv_params* p, int32_t seeds)
 
 int32_t asv::ctx_seed_cap(const asv_ctx* c) { return c ? c->seed_cap : 0; }
 
+
+
 extern "C" {
 
 int asv_seeds_placed(const asv_ctx* c) { return c ? c->d.A : fail(ASV_ERR_ARG, "null context"); }
@@ -629,8 +649,11 @@ int asv_seed_stage(asv_ctx* c, int32_t iter, void* stream)
     int gate = (iter >= 2 && c->p.angle_gate) ? 1 : 0;
     HIP_TRY(launch_seed_metric(st, c->d, c->pos, c->slab, c->fwd, c->bwd, gate, c->rec, c->seed_own, c->seed_need,
                                tr, c->unsafe));
-    HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
-    HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
+    // the fused first two passes read the seed positions, not a volume
+    if (c->pass_kind(0) != 3) {
+        HIP_TRY(launch_fill(st, c->labels[c->cur], c->V(), -1));
+        HIP_TRY(launch_place(st, c->d, c->pos, c->labels[c->cur]));
+    }
     c->fresh = true;
     c->last_iter = iter;
     return ASV_OK;
@@ -656,6 +679,13 @@ hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
     case 2:
         return launch_jfa_seeds2(st, c->d, in, out, c->lab, c->rec, c->pos, c->steps[0], c->steps[1], ratio,
                                  c->p.tie_rule, c->keys);
+    case 3:
+        // both passes: the second's output lands where it would after two
+        // passes, in the buffer this pass reads
+        return launch_seed_bricks(st, c->d, c->pos, c->rec, c->lab, c->labels[c->cur], c->steps[0], c->steps[1],
+                                  ratio, c->mult[1] * c->ratio, c->bricks);
+    case 4:
+        return hipSuccess;
     default:
         return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule, c->unsafe,
                           c->march_stats);
@@ -671,6 +701,23 @@ int asv_flood(asv_ctx* c, int32_t iter, void* stream)
     hipStream_t st = S(stream);
     for (size_t i = 0; i < c->steps.size(); ++i) {
         HIP_TRY(launch_pass(c, i, st));
+        c->cur = 1 - c->cur;
+    }
+    c->fresh = false;
+    return ASV_OK;
+}
+
+int asv_flood_passes(asv_ctx* c, int32_t iter, int32_t npasses, void* stream)
+{
+    if (!c) return fail(ASV_ERR_ARG, "null context");
+    if (iter != c->last_iter || !c->fresh)
+        return fail(ASV_ERR_STATE, "partial flood only right after the seed stage of this iteration");
+    if (npasses < 1 || npasses > (int)c->steps.size()) return fail(ASV_ERR_ARG, "pass count outside the schedule");
+    if (npasses == 1 && pass_kind_now(c, 0) == 3)
+        return fail(ASV_ERR_ARG, "the first two passes run fused: stop after the second");
+    hipStream_t st = S(stream);
+    for (int i = 0; i < npasses; ++i) {
+        HIP_TRY(launch_pass(c, (size_t)i, st));
         c->cur = 1 - c->cur;
     }
     c->fresh = false;

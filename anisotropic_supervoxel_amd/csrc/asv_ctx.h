@@ -45,6 +45,7 @@ struct asv_ctx {
     int64_t post_cap = 0;
     unsigned long long* ck_word = nullptr;
     unsigned long long* keys = nullptr;     // V keys for the scatter form of the sparse second pass
+    asv::SeedBricks bricks;                 // brick lists of the fused first two passes (launch_seed_bricks)
     int* unsafe = nullptr;                  // raised when a seed metric needs the exact clamp (JfaClip::unsafe)
     bool force_exact = false;               // asv_force_exact_clamp: raise it at every prepare
     unsigned long long* march_stats = nullptr;   // asv_enable_march_stats: JfaClip::stats
@@ -60,8 +61,25 @@ struct asv_ctx {
     // How pass i of an iteration that began with the seed stage runs: 1 the
     // first pass over the scattered seeds alone, 2 the second pass over the
     // first's output when it is sparse (about 729 voxels per seed against
-    // a third of the volume), 0 the full marching/tiled pass.
-    int pass_kind(size_t i) const { return asv::seed_pass_kind(steps, d.A, V(), i); }
+    // a third of the volume), 0 the full marching/tiled pass; 3 and 4 where
+    // the two sparse passes run fused from the seed positions (the first
+    // pass launches both and writes the second's output, the second
+    // launches nothing).
+    int pass_kind(size_t i) const
+    {
+        if (i < 2 && fused_seed_passes()) return i == 0 ? 3 : 4;
+        return asv::seed_pass_kind(steps, d.A, V(), i);
+    }
+
+    // Whether the first two passes run as launch_seed_bricks: the lowest-id
+    // rule, a first step of at most kSeedFuseMaxK1, a reach the brick lists
+    // were sized for and a sparse second pass.
+    bool fused_seed_passes() const
+    {
+        return bricks.entries != nullptr && p.tie_rule == ASV_TIE_LOWEST_ID && steps.size() > 1 &&
+               steps[0] <= asv::kSeedFuseMaxK1 && steps[0] + steps[1] <= bricks.r_cap &&
+               asv::seed_pass_kind(steps, d.A, V(), 1) == 2;
+    }
 
     // Whether the first pass must fill its output with -1: not when the
     // second pass is the scatter form, which reads the first pass's output

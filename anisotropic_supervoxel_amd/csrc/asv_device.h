@@ -91,6 +91,30 @@ __device__ __forceinline__ unsigned long long key_min(unsigned long long a, unsi
         "v"(__longlong_as_double((long long)b)));
     return (unsigned long long)__double_as_longlong(r);
 }
+// Whether coordinate c has coordinate a among its taps c - k, c, c + k
+// (each clamped to [0, n - 1]) along one axis.
+__device__ __forceinline__ bool axis_hit(int c, int a, int n, int k)
+{
+    return clampi(c - k, 0, n - 1) == a || c == a || clampi(c + k, 0, n - 1) == a;
+}
+
+// The idx-th coordinate v in [0, n) (ascending) whose taps v - k, v, v + k,
+// clamped to [0, n - 1], include p; false when there are fewer.  Such v lie
+// within k of p: v = p - k o exactly, or the clamped tap at an edge p = 0
+// (v <= k) or p = n - 1 (v >= n - 1 - k).
+__device__ __forceinline__ bool seed_axis(int p, int n, int k, int idx, int* v)
+{
+    const int lo = max(0, p - k), hi = min(n - 1, p + k);
+    for (int c = lo; c <= hi; ++c) {
+        const bool hit = clampi(c - k, 0, n - 1) == p || c == p || clampi(c + k, 0, n - 1) == p;
+        if (hit && idx-- == 0) {
+            *v = c;
+            return true;
+        }
+    }
+    return false;
+}
+
 #define ASV_KEY_INIT (((unsigned long long)0x4CBEBC20u) << 32)   /* bits of 1e8f */
 
 }  // namespace asv

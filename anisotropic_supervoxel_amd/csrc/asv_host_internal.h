@@ -11,6 +11,11 @@ namespace asv {
 bool ctx_reusable(const asv_ctx* c, const asv_params* p, int32_t seeds);
 int32_t ctx_seed_cap(const asv_ctx* c);
 
+// Device bytes of the brick lists of the fused first two passes for a clip
+// of W x H x D voxels with room for `seeds` seeds and a reach (first two
+// steps together) of at most R voxels.
+int64_t seed_brick_bytes(int W, int H, int D, int64_t seeds, int R);
+
 // A stream's resident context: taken out before the stream is destroyed,
 // offered to a new stream before its first block.  The stream adopts it
 // (true) when it fits; otherwise the caller keeps it.

@@ -100,6 +100,32 @@ hipError_t launch_jfa_seeds2(hipStream_t st, const Dims& d, const int32_t* in, i
                              unsigned long long* keys);
 hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n);
 
+// The first two passes of an iteration (steps k1 <= kSeedFuseMaxK1 and
+// k2 <= kSeedPassMaxStep, lowest-id rule) straight from the seed positions,
+// over bricks of 32 x 16 x kSeedBrickZ voxels (asv_seedpass.hip): `out`
+// receives the second pass's labels for every voxel, so neither the seed
+// stage's volume nor the first pass's output is needed.  The brick buffers
+// (alloc_seed_bricks, for clips of up to W x H x D voxels, `seeds` seeds
+// and a reach k1 + k2 of at most R): per-brick counts (zero between uses),
+// offsets and cursors, and the entries of every brick's seed list.
+constexpr int kSeedFuseMaxK1 = 2;
+constexpr int kSeedBrickZ = 16;
+struct SeedBricks {
+    int* count = nullptr;
+    int* off = nullptr;
+    int* cursor = nullptr;
+    void* entries = nullptr;
+    int64_t nb_cap = 0;
+    int seed_cap = 0;
+    int r_cap = 0;
+};
+void seed_brick_grid(int W, int H, int D, int* nbx, int* nby, int* nbz);
+int64_t seed_brick_bytes(int W, int H, int D, int64_t seeds, int R);
+hipError_t alloc_seed_bricks(SeedBricks* bb, int W, int H, int D, int seeds, int R, int64_t* total);
+void free_seed_bricks(SeedBricks* bb);
+hipError_t launch_seed_bricks(hipStream_t st, const Dims& d, const int4* pos, const SeedRec* rec, const uint32_t* lab,
+                              int32_t* out, int k1, int k2, float ratio1, float ratio2, const SeedBricks& bb);
+
 // How pass i of an iteration that began with the seed stage runs, for a
 // schedule `steps`, A seeds and V voxels: 1 the first pass over the
 // scattered seeds alone, when their neighbourhoods (27 voxels per seed at

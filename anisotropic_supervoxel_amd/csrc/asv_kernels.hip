@@ -768,23 +768,6 @@ __global__ __launch_bounds__(256) void k_jfa(int W, int H, int D, const int32_t*
     out[((size_t)z * H + y) * W + x] = jfa_voxel<TIE>(W, H, D, in, lab, rec, k, ratio, x, y, z);
 }
 
-// The idx-th coordinate v in [0, n) (ascending) whose taps v - k, v, v + k,
-// clamped to [0, n - 1], include p; false when there are fewer.  Such v lie
-// within k of p: v = p - k o exactly, or the clamped tap at an edge p = 0
-// (v <= k) or p = n - 1 (v >= n - 1 - k).
-__device__ __forceinline__ bool seed_axis(int p, int n, int k, int idx, int* v)
-{
-    const int lo = max(0, p - k), hi = min(n - 1, p + k);
-    for (int c = lo; c <= hi; ++c) {
-        const bool hit = clampi(c - k, 0, n - 1) == p || c == p || clampi(c + k, 0, n - 1) == p;
-        if (hit && idx-- == 0) {
-            *v = c;
-            return true;
-        }
-    }
-    return false;
-}
-
 // First pass of an iteration, over a volume holding only the scattered
 // seeds (every other voxel -1, as the seed stage leaves it): a voxel finds a
 // label among its 27 clamped taps only if some seed voxel p is one of them,
@@ -824,10 +807,6 @@ __global__ __launch_bounds__(256) void k_jfa_seeds(int W, int H, int D, int A, i
 // same value.
 constexpr int kSeed2Span = 2 * 2 * kSeedPassMaxStep + 1;   // coordinates within k1 + k2 of the seed
 
-__device__ __forceinline__ bool axis_hit(int c, int a, int n, int k)
-{
-    return clampi(c - k, 0, n - 1) == a || c == a || clampi(c + k, 0, n - 1) == a;
-}
 
 template <int TIE>
 __global__ __launch_bounds__(256) void k_jfa_seeds2(int W, int H, int D, int k1, int k2,

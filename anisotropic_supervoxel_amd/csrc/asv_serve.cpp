@@ -95,8 +95,10 @@ int64_t asv_serve_footprint(const asv_serve_params* sp, const asv_serve_request*
     for (int b = 0; b < nb && b < 1024; ++b) depth = std::max(depth, iv[2 * b + 1] - iv[2 * b]);
     const int64_t V = (int64_t)r->width * r->height * depth;
     const int64_t seeds = 4 * ((int64_t)p.seeds_per_100_frames * p.block_size / 100) + 64;
-    // frames, flows, LAB, two label volumes; seeds
-    return V * (3 + 16 + 4 + 8) + seeds * 240 + (1 << 20);
+    // frames, flows, LAB, two label volumes; seeds; the brick lists of the
+    // fused first passes where the window's seeds are sparse enough for them
+    const int64_t bricks = seeds * 729 * 3 <= V ? asv::seed_brick_bytes(r->width, r->height, depth, seeds, 2 + 16) : 0;
+    return V * (3 + 16 + 4 + 8) + seeds * 240 + bricks + (1 << 20);
 }
 
 int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int32_t n, asv_serve_result* results,

@@ -137,6 +137,12 @@ int asv_seed_stage(asv_ctx* ctx, int32_t iter, void* stream);
 
 /* Image::flood (GPU-AS/Image.cpp:1061-1190): the P JFA passes. */
 int asv_flood(asv_ctx* ctx, int32_t iter, void* stream);
+/* The first npasses passes of iteration iter's schedule only, right after
+ * its seed stage; asv_download_labels then returns pass npasses' output.
+ * Where the first two passes run fused (lowest-id rule, sparse seeds),
+ * npasses = 1 is refused.  Tests and tools; no counterpart in the
+ * reference. */
+int asv_flood_passes(asv_ctx* ctx, int32_t iter, int32_t npasses, void* stream);
 
 /* Image::centroids_next_seeds (GPU-AS/Image.cpp:1238-1297): reduction, then
  * centroid finalisation unless iter is the last iteration. */

@@ -322,6 +322,11 @@ int32_t asv::ctx_seed_cap(const asv_ctx* c)
     return c->seed_cap;
 }
 
+int64_t asv::seed_brick_bytes(int W, int H, int D, int64_t seeds, int)
+{
+    return (int64_t)((W + 31) / 32) * ((H + 15) / 16) * ((D + 15) / 16) * 12 + 4 + seeds * 48 * 64;
+}
+
 // ---- the harness ----------------------------------------------------------
 namespace {
 
