@@ -25,6 +25,20 @@ __device__ __forceinline__ float4 ld_f4(gptr<const float> p)
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// A load or store at a wave-uniform base plus a 32-bit byte offset per lane
+// (the base formed once per wave in scalar registers, one 64-bit add per
+// access instead of the full index arithmetic).
+template <typename T>
+__device__ __forceinline__ T ld_off(gptr<const T> base, uint32_t byte_off)
+{
+    return *(gptr<const T>)((gptr<const char>)base + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_off(gptr<T> base, uint32_t byte_off, T v)
+{
+    *(gptr<T>)((gptr<char>)base + byte_off) = v;
+}
+
 // Side of the square column of voxels a JFA block works on.
 constexpr int kTile = 16;
 

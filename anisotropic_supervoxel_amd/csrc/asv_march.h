@@ -209,15 +209,20 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         // a slab is staged in raster order over the grown tile: thread tid
         // holds elements tid, tid + 256, ... (offsets within a frame, -1 for
         // none), so every lane of every wave stages
+        // (byte offsets within a frame: a load takes the frame's base in
+        // scalar registers and this offset)
         int soff[NRS];
 #pragma unroll
         for (int i = 0; i < NRS; ++i) {
             const int f = tid + 256 * i;
             const int rr = f / R, cc = f - (f / R) * R;
-            soff[i] = f < RA ? clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1) : -1;
+            soff[i] = f < RA ? 4 * (clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1)) : -1;
         }
         const int x = x0 + lx, y = y0 + ly;
         const bool mine = x < W && y < H;
+        // this thread's voxel within a frame (clamped into it), in bytes
+        const uint32_t voff = 4u * (uint32_t)(min(y, H - 1) * W + min(x, W - 1));
+        const size_t plane = (size_t)H * W;
         const float fx = (float)x, fy = (float)y;
         // A clip whose seed stage raised `unsafe` (a metric that could make a
         // quadratic form NaN) takes the per-tap path at every step, which
@@ -225,15 +230,15 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const bool exact = c.unsafe && *(gptr<const int>)c.unsafe;
 
         auto load_slab = [&](int gz, int* ids) {
-            const gptr<const int32_t> slab = g_in + (size_t)gz * H * W;
+            const gptr<const int32_t> slab = g_in + (size_t)gz * plane;
             // kStraight: cells past the grown tile read cell 0 and drop it
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
                 if (kStraight) {
-                    const int v = slab[soff[i] >= 0 ? soff[i] : 0];
+                    const int v = ld_off(slab, (uint32_t)max(soff[i], 0));
                     ids[i] = soff[i] >= 0 ? v : -1;
                 } else {
-                    ids[i] = soff[i] >= 0 ? slab[soff[i]] : -1;
+                    ids[i] = soff[i] >= 0 ? ld_off(slab, (uint32_t)soff[i]) : -1;
                 }
             }
         };
@@ -405,7 +410,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             pk_next[k] = 0u;
-            if (kLabAhead && mine && k < n) pk_next[k] = g_lab[((size_t)(z + k * K) * H + y) * W + x];
+            if (kLabAhead && mine && k < n) pk_next[k] = ld_off(g_lab + (size_t)(z + k * K) * plane, voff);
         }
         bool full = true;
         // ST: the phase clock of this wave (uniform)
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             for (int k = 0; k < S; ++k) {
                 pk_cur[k] = 0u;
                 if (!kLabAhead)
-                    pk_cur[k] = g_lab[((size_t)min(z + k * K, D - 1) * H + min(y, H - 1)) * W + min(x, W - 1)];
+                    pk_cur[k] = ld_off(g_lab + (size_t)min(z + k * K, D - 1) * plane, voff);
             }
             if (full) {
                 release();
@@ -519,9 +524,9 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     if (kLabAhead && kStraight)
-                        pk_next[k] = g_lab[((size_t)min(zn + k * K, D - 1) * H + min(y, H - 1)) * W + min(x, W - 1)];
+                        pk_next[k] = ld_off(g_lab + (size_t)min(zn + k * K, D - 1) * plane, voff);
                     else if (kLabAhead && mine && step + S + k < n)
-                        pk_next[k] = g_lab[((size_t)(zn + k * K) * H + y) * W + x];
+                        pk_next[k] = ld_off(g_lab + (size_t)(zn + k * K) * plane, voff);
                 }
             }
 
@@ -556,7 +561,6 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 for (int k = 0; k < S; ++k) {
                     if (k >= ns) break;
                     const int zk = z + k * K;
-                    const size_t v = ((size_t)zk * H + y) * W + x;
                     float l0, l1, l2;
                     lab_of(pk_here[k], l0, l1, l2);
                     unsigned long long best = ASV_KEY_INIT;
@@ -608,7 +612,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             }
                         }
                     }
-                    g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
+                    st_off(g_out + (size_t)zk * plane, voff,
+                           best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
                 }
             }
             if (kWide && nchunk > 1) {
@@ -631,8 +636,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                                 kw[0] = (uint32_t)best;
                                 kw[256] = (uint32_t)(best >> 32);
                             } else {
-                                const size_t v = ((size_t)(z + k * K) * H + y) * W + x;
-                                g_out[v] = best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull);
+                                st_off(g_out + (size_t)(z + k * K) * plane, voff,
+                                       best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
                             }
                         }
                     }

@@ -145,11 +145,48 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
 
     // A new stream adopts an idle context of its frame size and settings
     // when one is there (no device allocation on admission).
-    // Each worker delivers a window's finished frames into its own
-    // page-locked buffer (one transfer), grown to the largest window seen.
+    // Each worker owns page-locked buffers sized once for the largest window
+    // of the run: the window's finished labels (one transfer out) and the
+    // generated frames and flows of a window (direct transfers in).  No
+    // worker allocates or frees page-locked memory while others copy.
+    size_t max_window = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!well_formed(&reqs[i])) continue;
+        const asv_stream_params q = stream_params_of(sp, &reqs[i]);
+        const int nbq = asv_stream_plan(&q, nullptr, 0);
+        if (nbq < 1) continue;
+        std::vector<int32_t> ivq((size_t)nbq * 2);
+        asv_stream_plan(&q, ivq.data(), nbq);
+        for (int b = 0; b < nbq; ++b)
+            max_window = std::max(max_window, (size_t)(ivq[2 * (size_t)b + 1] - ivq[2 * (size_t)b]) *
+                                                  (size_t)reqs[i].width * (size_t)reqs[i].height);
+    }
     struct WindowOut {
         int32_t* p = nullptr;
+        uint8_t* rgb = nullptr;
+        float* fwd = nullptr;
+        float* bwd = nullptr;
         size_t cap = 0;
+        bool ok = false;
+        void alloc(size_t voxels)
+        {
+            cap = voxels;
+            ok = hipHostMalloc((void**)&p, voxels * sizeof(int32_t), hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void**)&rgb, voxels * 3, hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void**)&fwd, voxels * 2 * sizeof(float), hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void**)&bwd, voxels * 2 * sizeof(float), hipHostMallocDefault) == hipSuccess;
+        }
+        void release()
+        {
+            void* ptrs[] = {p, rgb, fwd, bwd};
+            for (void* q : ptrs)
+                if (q) (void)hipHostFree(q);
+            p = nullptr;
+            rgb = nullptr;
+            fwd = bwd = nullptr;
+            cap = 0;
+            ok = false;
+        }
     };
 
     // Windows handed to the batcher: the context whose loop to run, the
@@ -238,16 +275,9 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         const size_t plane = (size_t)r.width * r.height;
         size_t max_nf = 0;
         for (int b = 0; b < nb; ++b) max_nf = std::max(max_nf, (size_t)(iv[2 * (size_t)b + 1] - iv[2 * (size_t)b]));
-        if (max_nf * plane > wo.cap) {
-            if (wo.p) (void)hipHostFree(wo.p);
-            wo.p = nullptr;
-            wo.cap = 0;
-            if (hipHostMalloc((void**)&wo.p, max_nf * plane * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
-                wo.p = nullptr;
-                asv_stream_destroy(s);
-                return ASV_ERR_NOMEM;
-            }
-            wo.cap = max_nf * plane;
+        if (!wo.ok || max_nf * plane > wo.cap) {
+            asv_stream_destroy(s);
+            return ASV_ERR_NOMEM;
         }
         uint64_t vox_iters = 0;
         double ready = ms_since(t0);
@@ -261,8 +291,6 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             }
             const int f0 = iv[2 * (size_t)b], f1 = iv[2 * (size_t)b + 1];
             const size_t nf = (size_t)(f1 - f0);
-            std::vector<uint8_t> rgb_buf;
-            std::vector<float> fwd_buf, bwd_buf;
             const uint8_t* rgb;
             const float *fwd, *bwd;
             if (r.rgb && r.fwd && r.bwd) {
@@ -270,16 +298,12 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
                 fwd = r.fwd + (size_t)f0 * plane * 2;
                 bwd = r.bwd + (size_t)f0 * plane * 2;
             } else {
-                rgb_buf.resize(plane * nf * 3);
-                fwd_buf.resize(plane * nf * 2);
-                bwd_buf.resize(plane * nf * 2);
                 rc = asv_synth_video_range(r.video_seed, r.width, r.height, r.frames, r.n_ellipses, r.vmax,
-                                           r.noise_sigma, f0, f1, rgb_buf.data(), fwd_buf.data(), bwd_buf.data(),
-                                           nullptr);
+                                           r.noise_sigma, f0, f1, wo.rgb, wo.fwd, wo.bwd, nullptr);
                 if (rc) break;
-                rgb = rgb_buf.data();
-                fwd = fwd_buf.data();
-                bwd = bwd_buf.data();
+                rgb = wo.rgb;
+                fwd = wo.fwd;
+                bwd = wo.bwd;
             }
             // the window is ready once its frames are present and the previous window is done
             ready = std::max(ready, ms_since(t0));
@@ -337,6 +361,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
         hipStream_t st = nullptr;
         (void)hipStreamCreate(&st);
         WindowOut wo;
+        wo.alloc(max_window);
         for (;;) {
             int ri;
             {
@@ -356,7 +381,7 @@ int asv_serve_run(const asv_serve_params* sp, const asv_serve_request* reqs, int
             }
             cv.notify_all();
         }
-        if (wo.p) (void)hipHostFree(wo.p);
+        wo.release();
         if (st) (void)hipStreamDestroy(st);
     };
 
