@@ -62,6 +62,7 @@ struct PassGroup {
     bool ht = false;               // a clip has more seeds than the marching table has slots
     bool counted = false;          // a clip keeps marching-step counters
     bool dense = false;            // few voxels per seed over the clips (march_dense)
+    int lt = 0;                    // the label widths of the group's clips (launch_jfa)
 };
 
 JfaClip descriptor(const asv_ctx* c, int p)
@@ -103,16 +104,18 @@ void build_groups(const std::vector<asv_ctx*>& clips, std::vector<PassGroup>* gr
             // passes over the seeds' neighbourhoods alone run per clip
             const bool march = c->pass_kind((size_t)p) == 0 && K <= 16 && c->p.tie_rule == ASV_TIE_LOWEST_ID &&
                                c->d.A <= kMarchMaxIds;
+            const int lt = c->label_widths((size_t)p, c->steps.size(), true);
             PassGroup* g = nullptr;
             if (march)
                 for (PassGroup& q : row)
-                    if (q.batched && q.K == K) g = &q;
+                    if (q.batched && q.K == K && q.lt == lt) g = &q;
             if (!g) {
                 row.emplace_back();
                 g = &row.back();
                 g->p = p;
                 g->K = K;
                 g->batched = march;
+                g->lt = lt;
             }
             g->clips.push_back(i);
             if (march) {
@@ -136,7 +139,7 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
 {
     if (g.batched && !g.split) {
         BHIP(launch_march_batch(st, g.K, dev_desc + g.dev_off, (int)g.desc.size(), g.grid, g.nitems, g.dyn, g.ht,
-                                g.counted, g.dense));
+                                g.counted, g.dense, g.lt));
         return ASV_OK;
     }
     for (int i : g.clips) {
@@ -160,7 +163,8 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
         case 4:                                     // done by pass 0
             break;
         default:
-            BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe, c->march_stats));
+            BHIP(launch_jfa(st, c->d, in, out, c->lab, c->rec, g.K, ratio, c->p.tie_rule, c->unsafe, c->march_stats,
+                            c->label_widths((size_t)g.p, c->steps.size(), true)));
             break;
         }
     }
@@ -220,7 +224,7 @@ namespace {
 int upload_group(asv_batch* b, PassGroup& g)
 {
     g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), g.per_block, g.grid_limit, g.seg_cap, &g.nitems,
-                        &g.dyn);
+                        &g.dyn, g.lt);
     if (g.grid <= 0) return bfail(ASV_ERR_ARG, "no marching kernel for this step");
     BHIP(hipMemcpy(b->dev_desc + g.dev_off, g.desc.data(), sizeof(JfaClip) * g.desc.size(), hipMemcpyHostToDevice));
     return ASV_OK;
@@ -252,7 +256,7 @@ int autotune(asv_batch* b)
     float total = 0.0f;
     for (PassGroup& g : b->groups) {
         if (g.batched) {
-            const int gmax = march_max_grid(g.K, g.dyn, g.ht, g.dense);
+            const int gmax = march_max_grid(g.K, g.dyn, g.ht, g.dense, g.lt);
             struct Shape {
                 int gl, pb, cap;
                 bool split;
@@ -516,7 +520,7 @@ int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, v
     build_groups(clips, &groups, &n_desc);
     for (PassGroup& g : groups)
         if (g.batched) {
-            g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), 0, 0, 0, &g.nitems, &g.dyn);
+            g.grid = march_plan(g.K, g.desc.data(), (int)g.desc.size(), 0, 0, 0, &g.nitems, &g.dyn, g.lt);
             if (g.grid <= 0) g.split = true;      // each context's own launch
         }
     Impl::Slot& sl = im.ring[im.next];
@@ -571,7 +575,7 @@ int asv::WindowBatcher::run(asv_ctx* const* ctxs, void* const* streams, int n, v
                     if (!rc) rc = to_batch(i);
                 if (!rc) {
                     const hipError_t e = launch_march_batch(bst, g.K, sl.dev + g.dev_off, (int)g.desc.size(), g.grid,
-                                                            g.nitems, g.dyn, g.ht, g.counted, g.dense);
+                                                            g.nitems, g.dyn, g.ht, g.counted, g.dense, g.lt);
                     if (e != hipSuccess) rc = bfail(ASV_ERR_HIP, "marching launch", hipGetErrorString(e));
                     batch_ahead = true;
                 }

@@ -667,7 +667,8 @@ namespace {
 // carry a label (asv_ctx::pass_kind).
 int pass_kind_now(const asv_ctx* c, size_t i) { return c->fresh ? c->pass_kind(i) : 0; }
 
-hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
+// (n: the passes this flood runs, whose last writes 32-bit labels)
+hipError_t launch_pass(asv_ctx* c, size_t i, size_t n, hipStream_t st)
 {
     const float ratio = c->mult[i] * c->ratio;
     const int32_t* in = c->labels[c->cur];
@@ -688,7 +689,7 @@ hipError_t launch_pass(asv_ctx* c, size_t i, hipStream_t st)
         return hipSuccess;
     default:
         return launch_jfa(st, c->d, in, out, c->lab, c->rec, c->steps[i], ratio, c->p.tie_rule, c->unsafe,
-                          c->march_stats);
+                          c->march_stats, c->label_widths(i, n, c->fresh));
     }
 }
 
@@ -700,7 +701,7 @@ int asv_flood(asv_ctx* c, int32_t iter, void* stream)
     if (iter != c->last_iter) return fail(ASV_ERR_STATE, "flood before the seed stage of this iteration");
     hipStream_t st = S(stream);
     for (size_t i = 0; i < c->steps.size(); ++i) {
-        HIP_TRY(launch_pass(c, i, st));
+        HIP_TRY(launch_pass(c, i, c->steps.size(), st));
         c->cur = 1 - c->cur;
     }
     c->fresh = false;
@@ -717,7 +718,7 @@ int asv_flood_passes(asv_ctx* c, int32_t iter, int32_t npasses, void* stream)
         return fail(ASV_ERR_ARG, "the first two passes run fused: stop after the second");
     hipStream_t st = S(stream);
     for (int i = 0; i < npasses; ++i) {
-        HIP_TRY(launch_pass(c, (size_t)i, st));
+        HIP_TRY(launch_pass(c, (size_t)i, (size_t)npasses, st));
         c->cur = 1 - c->cur;
     }
     c->fresh = false;
@@ -785,7 +786,7 @@ int asv_run_timed(asv_ctx* c, void* stream, float* ms, int32_t* passes)
         for (size_t i = 0; i < c->steps.size(); ++i) {
             a = mark();
             const bool first = pass_kind_now(c, i) != 0;
-            HIP_TRY(launch_pass(c, i, st));
+            HIP_TRY(launch_pass(c, i, c->steps.size(), st));
             spans.push_back({a, mark(), first ? 5 : 2});
             c->cur = 1 - c->cur;
             if (!first) ++np;

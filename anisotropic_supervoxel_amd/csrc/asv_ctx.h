@@ -81,6 +81,23 @@ struct asv_ctx {
                asv::seed_pass_kind(steps, d.A, V(), 1) == 2;
     }
 
+    // 16-bit labels between marching passes (launch_jfa's lt): pass i of a
+    // flood of n passes writes them where the next pass of the same flood
+    // marches in a form that reads them, so only the flood's last pass
+    // writes 32-bit labels.  `fresh_iter`: the passes of an iteration that
+    // began with the seed stage (pass_kind), else every pass is a full one.
+    bool pass_label16(size_t i, bool fresh_iter) const
+    {
+        return (fresh_iter ? pass_kind(i) : 0) == 0 &&
+               asv::march_label16(d, steps[i], p.tie_rule, march_stats != nullptr);
+    }
+    int label_widths(size_t i, size_t n, bool fresh_iter) const
+    {
+        const bool out16 = i + 1 < n && pass_label16(i, fresh_iter) && pass_label16(i + 1, fresh_iter);
+        const bool in16 = i > 0 && i < n && pass_label16(i - 1, fresh_iter) && pass_label16(i, fresh_iter);
+        return in16 ? (out16 ? 2 : 3) : (out16 ? 1 : 0);
+    }
+
     // Whether the first pass must fill its output with -1: not when the
     // second pass is the scatter form, which reads the first pass's output
     // only at the voxels the first pass writes (the seeds' step-1 sets).

@@ -79,9 +79,17 @@ hipError_t launch_fill(hipStream_t st, int32_t* p, int64_t n, int32_t value);
 hipError_t launch_zero_u64(hipStream_t st, uint64_t* p, int64_t n);
 hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
 hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
+// lt: the label widths of a marching pass (0: 32-bit in and out; 1 32-bit
+// in, 16-bit out; 2 16-bit in and out; 3 16-bit in, 32-bit out; 16-bit
+// labels are 0xFFFF for -1), only where march_label16 allows them.
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
-                      unsigned long long* stats = nullptr);
+                      unsigned long long* stats = nullptr, int lt = 0);
+// Whether launch_jfa runs a pass of step k over d as a marching kernel
+// with 16-bit label forms: the lowest-id rule, the direct id table (at most
+// kMarchTableIds seeds, so every id fits 16 bits), two slices per step, an
+// uncounted launch, and a clip small enough that its one-clip plan holds.
+bool march_label16(const Dims& d, int k, int tie_rule, bool counted);
 // The first pass of an iteration when `in` holds only the seeds scattered
 // by the seed stage at `pos` (every other voxel -1): fills `out` with -1
 // and evaluates only the voxels that have a seed among their taps, at most
@@ -161,10 +169,11 @@ constexpr int kMarchMaxIds = (1 << 30) - 1;  // ids below the resolved-index tag
 // selects the wider candidate masks (march_mc).
 bool march_dense(const JfaClip* c, int n);
 int march_mc(int k, bool dense = false);
-int march_max_grid(int k, int dyn, bool ht = false, bool dense = false);
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn);
+int march_max_grid(int k, int dyn, bool ht = false, bool dense = false, int lt = 0);
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn,
+               int lt = 0);
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht, bool counted = false, bool dense = false);
+                              bool ht, bool counted = false, bool dense = false, int lt = 0);
 
 hipError_t launch_reduce(hipStream_t st, const Dims& d, const int32_t* labels, const uint32_t* lab,
                          unsigned long long* sums);

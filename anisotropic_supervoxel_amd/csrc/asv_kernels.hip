@@ -1508,13 +1508,13 @@ int march_s(int k)
 // The marching kernel for step k, candidate capacity mc, slices per step sl;
 // ht: some clip of the launch has more seeds than the table has slots;
 // counted: the counting form.
-static MarchFn march_kernel(int k, int mc, int sl, bool ht = false, bool counted = false)
+static MarchFn march_kernel(int k, int mc, int sl, bool ht = false, bool counted = false, int lt = 0)
 {
     switch ((k - 1) >> 2) {
-    case 0: return march_kernel_part<0>(k, mc, sl, ht, counted);
-    case 1: return march_kernel_part<1>(k, mc, sl, ht, counted);
-    case 2: return march_kernel_part<2>(k, mc, sl, ht, counted);
-    case 3: return march_kernel_part<3>(k, mc, sl, ht, counted);
+    case 0: return march_kernel_part<0>(k, mc, sl, ht, counted, lt);
+    case 1: return march_kernel_part<1>(k, mc, sl, ht, counted, lt);
+    case 2: return march_kernel_part<2>(k, mc, sl, ht, counted, lt);
+    case 3: return march_kernel_part<3>(k, mc, sl, ht, counted, lt);
     default: return nullptr;
     }
 }
@@ -1530,18 +1530,19 @@ static int march_dyn_bytes(int amax)
 
 // Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
 // 8), cached per step, capacity and table size.
-int march_max_grid(int k, int dyn, bool ht, bool dense)
+int march_max_grid(int k, int dyn, bool ht, bool dense, int lt)
 {
-    static int cache[17][2][2][2][kMarchTableIds / 1024 + 1];
-    if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchTableIds) return 0;
+    static int cache[17][2][2][2][4][kMarchTableIds / 1024 + 1];
+    if (k < 1 || k > 16 || dyn < 0 || dyn > kMarchTableIds || lt < 0 || lt > 3) return 0;
     const int mc = march_mc(k, dense), sl = march_s(k);
-    int& slot = cache[k][mc == 63][sl == 2][ht][dyn / 1024];
+    if (lt != 0 && !march_kernel(k, mc, sl, ht, false, lt)) return 0;
+    int& slot = cache[k][mc == 63][sl == 2][ht][lt][dyn / 1024];
     int g = __atomic_load_n(&slot, __ATOMIC_RELAXED);
     if (g == 0) {
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl, ht), 256,
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl, ht, false, lt), 256,
                                                            (size_t)dyn);
         g = cus * (per_cu > 0 ? per_cu : 1);
         g = (g / 8) * 8;
@@ -1579,14 +1580,15 @@ static int64_t plan_items(int k, JfaClip* c, int n, int64_t want, int seg_cap, i
     return items;
 }
 
-int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn)
+int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_cap, int* nitems, int* dyn, int lt)
 {
     if (k < 1 || k > 16 || n < 1) return 0;
     int amax = 1;
     for (int i = 0; i < n; ++i) amax = c[i].A > amax ? c[i].A : amax;
     if (amax > kMarchMaxIds) return 0;
     const int lds = march_dyn_bytes(amax < kMarchTableIds ? amax : kMarchTableIds);
-    int gmax = march_max_grid(k, lds, amax > kMarchTableIds, march_dense(c, n));
+    int gmax = march_max_grid(k, lds, amax > kMarchTableIds, march_dense(c, n), lt);
+    if (gmax <= 0) return 0;
     if (grid_limit > 0 && grid_limit < gmax) gmax = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
     const int64_t pmax = gmax / 8;                       // resident blocks per XCD
     int64_t columns = 0;
@@ -1621,9 +1623,9 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
 }
 
 hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip* dev, int n, int grid, int nitems,
-                        int dyn, bool ht, bool counted, bool dense)
+                        int dyn, bool ht, bool counted, bool dense, int lt)
 {
-    MarchFn fn = march_kernel(k, march_mc(k, dense), march_s(k), ht, counted);
+    MarchFn fn = march_kernel(k, march_mc(k, dense), march_s(k), ht, counted, lt);
     if (!fn || grid < 8 || (grid & 7) || dyn < 4) return hipErrorInvalidValue;
     JfaClip a0 = one;
     const JfaClip* a1 = dev;
@@ -1633,25 +1635,26 @@ hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip
 }
 
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
-                              bool ht, bool counted, bool dense)
+                              bool ht, bool counted, bool dense, int lt)
 {
     JfaClip none{};
-    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht, counted, dense);
+    return launch_march(st, k, none, dev, n, grid, nitems, dyn, ht, counted, dense, lt);
 }
 
 template <int K>
 static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
-                              const SeedRec* rec, float ratio, const int* unsafe, unsigned long long* stats)
+                              const SeedRec* rec, float ratio, const int* unsafe, unsigned long long* stats, int lt)
 {
     int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     if (d.A <= kMarchMaxIds) {
         JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe, stats};
         int nitems = 0, dyn = 0;
-        const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn);
+        const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn, lt);
         if (g > 0)
             return launch_march(st, K, c, nullptr, 1, g, nitems, dyn, d.A > kMarchTableIds, stats != nullptr,
-                                march_dense(&c, 1));
+                                march_dense(&c, 1), lt);
     }
+    if (lt != 0) return hipErrorInvalidValue;   // 16-bit labels only between marching passes
     int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
                        rec, ratio);
@@ -1707,29 +1710,45 @@ hipError_t launch_fill_keys(hipStream_t st, unsigned long long* keys, int64_t n)
     return hipGetLastError();
 }
 
+bool march_label16(const Dims& d, int k, int tie_rule, bool counted)
+{
+    // ASV_LABEL16=0 keeps 32-bit labels between passes (tests and measurement)
+    static const bool off = [] {
+        const char* e = getenv("ASV_LABEL16");
+        return e && e[0] == '0';
+    }();
+    if (off) return false;
+    const int64_t tiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile);
+    // launch_tile's conditions for the marching kernel; the item bound is
+    // the largest a one-clip plan reaches (32 segments per column)
+    return tie_rule == 0 && k >= 1 && k <= 16 && tiles * d.D < (1ll << 31) && d.A <= kMarchTableIds &&
+           !counted && march_s(k) == 2 && tiles * k * 32 < 0x7fffffffll;
+}
+
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
-                      unsigned long long* stats)
+                      unsigned long long* stats, int lt)
 {
+    if (lt != 0 && !march_label16(d, k, tie_rule, stats != nullptr)) return hipErrorInvalidValue;
     int64_t ntiles = (int64_t)((d.W + kTile - 1) / kTile) * ((d.H + kTile - 1) / kTile) * d.D;
     if (tie_rule == 0 && ntiles < (1ll << 31)) {
         switch (k) {
-        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio, unsafe, stats);
-        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio, unsafe, stats);
+        case 1: return launch_tile<1>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 2: return launch_tile<2>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 3: return launch_tile<3>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 4: return launch_tile<4>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 5: return launch_tile<5>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 6: return launch_tile<6>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 7: return launch_tile<7>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 8: return launch_tile<8>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 9: return launch_tile<9>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 10: return launch_tile<10>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 11: return launch_tile<11>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 12: return launch_tile<12>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 13: return launch_tile<13>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 14: return launch_tile<14>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 15: return launch_tile<15>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
+        case 16: return launch_tile<16>(st, d, in, out, lab, rec, ratio, unsafe, stats, lt);
         default: break;
         }
     }

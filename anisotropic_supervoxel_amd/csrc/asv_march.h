@@ -90,10 +90,16 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 // ST: the counting form, which adds each item's steps, wide steps, their
 // chunks and per-tap steps to c.stats (when set); the kernels of ordinary
 // launches carry none of that code.
-template <int K, int MC, int S, bool HT, bool ST>
+//
+// I16 / O16: the input / output labels are 16-bit (0xFFFF for -1), for
+// clips of at most kLabel16Ids seeds between two marching passes: the
+// passes move 2 bytes per label instead of 4 (the slabs with their halos,
+// and the output), and only the last pass of a flood writes 32-bit labels.
+template <int K, int MC, int S, bool HT, bool ST, bool I16, bool O16>
 __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
                                                    int nitems, int slot_words)
 {
+    constexpr int kInB = I16 ? 2 : 4;         // bytes per input label
     static_assert(S == 1 || S == 2, "one or two slices of a chain per step");
     using MaskT = typename MarchMask<MC>::T;
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
@@ -155,8 +161,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     const int i_end = i_beg + qx + (xcd < rx ? 1 : 0);
 
     const int wave = tid >> 6, lane = tid & 63;
-    const int lx = (wave & 1) * 8 + (lane & 7);
-    const int ly = (wave >> 1) * 8 + (lane >> 3);
+    // a wave covers four full rows of the column (64-byte label rows,
+    // against 32-byte rows of an 8 x 8 wave: 318 -> 308 us per pass at
+    // step 1, 419 -> 411 at step 5, 640x480x200)
+    const int lx = lane & 15;
+    const int ly = wave * 4 + (lane >> 4);
     const int mbase = (ly + K) * R + (lx + K);
 
     int known = 0;                            // s_count as of the last barrier
@@ -198,8 +207,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const int W = c.W, H = c.H, D = c.D;
         // the descriptor's pointers are global memory (flat accesses would
         // share the LDS wait counter)
-        const gptr<const int32_t> g_in = (gptr<const int32_t>)c.in;
-        const gptr<int32_t> g_out = (gptr<int32_t>)c.out;
+        const gptr<const char> g_in = (gptr<const char>)c.in;
+        const gptr<char> g_out = (gptr<char>)c.out;
         const gptr<const uint32_t> g_lab = (gptr<const uint32_t>)c.lab;
         const gptr<const float> g_rec = (gptr<const float>)c.rec;
         const float ratio = c.ratio;
@@ -216,7 +225,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         for (int i = 0; i < NRS; ++i) {
             const int f = tid + 256 * i;
             const int rr = f / R, cc = f - (f / R) * R;
-            soff[i] = f < RA ? 4 * (clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1)) : -1;
+            soff[i] = f < RA ? kInB * (clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1)) : -1;
         }
         const int x = x0 + lx, y = y0 + ly;
         const bool mine = x < W && y < H;
@@ -229,16 +238,31 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         // clamps with cand_dist's exact rule (the flag is uniform per item)
         const bool exact = c.unsafe && *(gptr<const int>)c.unsafe;
 
+        // a label at a frame base plus a byte offset, and a label stored at
+        // this thread's voxel of frame zk
+        auto ld_label = [&](gptr<const char> base, uint32_t off) -> int {
+            if (I16) {
+                const unsigned v = ld_off((gptr<const uint16_t>)base, off);
+                return v == 0xFFFFu ? -1 : (int)v;
+            }
+            return ld_off((gptr<const int32_t>)base, off);
+        };
+        auto st_label = [&](int zk, int v) {
+            if (O16)
+                st_off((gptr<uint16_t>)(g_out + (size_t)zk * plane * 2), voff >> 1, (uint16_t)v);
+            else
+                st_off((gptr<int32_t>)(g_out + (size_t)zk * plane * 4), voff, (int32_t)v);
+        };
         auto load_slab = [&](int gz, int* ids) {
-            const gptr<const int32_t> slab = g_in + (size_t)gz * plane;
+            const gptr<const char> slab = g_in + (size_t)gz * plane * kInB;
             // kStraight: cells past the grown tile read cell 0 and drop it
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
                 if (kStraight) {
-                    const int v = ld_off(slab, (uint32_t)max(soff[i], 0));
+                    const int v = ld_label(slab, (uint32_t)max(soff[i], 0));
                     ids[i] = soff[i] >= 0 ? v : -1;
                 } else {
-                    ids[i] = soff[i] >= 0 ? ld_off(slab, (uint32_t)soff[i]) : -1;
+                    ids[i] = soff[i] >= 0 ? ld_label(slab, (uint32_t)soff[i]) : -1;
                 }
             }
         };
@@ -595,11 +619,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             const int zz = sl == 0 ? zk : clampi(zk + (sl == 1 ? -K : K), 0, D - 1);
 #pragma unroll 1
                             for (int rr = -1; rr <= 1; ++rr) {
-                                const gptr<const int32_t> row =
-                                    g_in + ((size_t)zz * H + clampi(y + rr * K, 0, H - 1)) * W;
+                                const gptr<const char> row =
+                                    g_in + ((size_t)zz * H + clampi(y + rr * K, 0, H - 1)) * W * kInB;
 #pragma unroll 1
                                 for (int cc = -1; cc <= 1; ++cc) {
-                                    const int id = row[clampi(x + cc * K, 0, W - 1)];
+                                    const int id = ld_label(row, (uint32_t)(clampi(x + cc * K, 0, W - 1) * kInB));
                                     if (id < 0 || id == seen0 || id == seen1) continue;
                                     seen1 = seen0;
                                     seen0 = id;
@@ -612,8 +636,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             }
                         }
                     }
-                    st_off(g_out + (size_t)zk * plane, voff,
-                           best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
+                    st_label(zk, best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
                 }
             }
             if (kWide && nchunk > 1) {
@@ -636,8 +659,8 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                                 kw[0] = (uint32_t)best;
                                 kw[256] = (uint32_t)(best >> 32);
                             } else {
-                                st_off(g_out + (size_t)(z + k * K) * plane, voff,
-                                       best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
+                                st_label(z + k * K,
+                                         best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));
                             }
                         }
                     }
@@ -679,14 +702,30 @@ using MarchFn = void (*)(JfaClip, const JfaClip*, int, int, int);
 template <int K, bool HT, bool ST>
 static MarchFn march_fn(int mc, int sl)
 {
-    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2, HT, ST> : k_jfa_march<K, 63, 2, HT, ST>;
-    return mc == 31 ? k_jfa_march<K, 31, 1, HT, ST> : k_jfa_march<K, 63, 1, HT, ST>;
+    if (sl == 2) return mc == 31 ? k_jfa_march<K, 31, 2, HT, ST, false, false> : k_jfa_march<K, 63, 2, HT, ST, false, false>;
+    return mc == 31 ? k_jfa_march<K, 31, 1, HT, ST, false, false> : k_jfa_march<K, 63, 1, HT, ST, false, false>;
+}
+
+// The 16-bit label forms exist for two slices per step, the direct table
+// and ordinary launches (lt: 1 32-bit in, 16-bit out; 2 both 16-bit; 3
+// 16-bit in, 32-bit out).
+template <int K>
+static MarchFn march_fn16(int mc, int lt)
+{
+    switch (lt) {
+    case 1: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, false, true> : k_jfa_march<K, 63, 2, false, false, false, true>;
+    case 2: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, true> : k_jfa_march<K, 63, 2, false, false, true, true>;
+    case 3: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, false> : k_jfa_march<K, 63, 2, false, false, true, false>;
+    default: return nullptr;
+    }
 }
 
 // Instances of steps 4P+1 .. 4P+4 (defined by asv_march.hip built with
 // ASV_MARCH_PART = P): the kernel for step k, capacity mc, slices per step
-// sl; ht selects the hashed-table form, counted the counting form.
+// sl; ht selects the hashed-table form, counted the counting form, lt the
+// label widths (0: 32-bit in and out; see march_fn16), null where no such
+// instance exists.
 template <int P>
-MarchFn march_kernel_part(int k, int mc, int sl, bool ht, bool counted);
+MarchFn march_kernel_part(int k, int mc, int sl, bool ht, bool counted, int lt);
 
 }  // namespace asv

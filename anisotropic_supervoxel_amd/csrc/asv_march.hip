@@ -12,8 +12,9 @@ namespace asv {
 namespace {
 
 template <int K>
-MarchFn pick(int mc, int sl, bool ht, bool counted)
+MarchFn pick(int mc, int sl, bool ht, bool counted, int lt)
 {
+    if (lt != 0) return (sl == 2 && !ht && !counted) ? march_fn16<K>(mc, lt) : nullptr;
     if (ht) return counted ? march_fn<K, true, true>(mc, sl) : march_fn<K, true, false>(mc, sl);
     return counted ? march_fn<K, false, true>(mc, sl) : march_fn<K, false, false>(mc, sl);
 }
@@ -21,14 +22,14 @@ MarchFn pick(int mc, int sl, bool ht, bool counted)
 }  // namespace
 
 template <>
-MarchFn march_kernel_part<ASV_MARCH_PART>(int k, int mc, int sl, bool ht, bool counted)
+MarchFn march_kernel_part<ASV_MARCH_PART>(int k, int mc, int sl, bool ht, bool counted, int lt)
 {
     constexpr int k0 = 4 * ASV_MARCH_PART;
     switch (k - k0) {
-    case 1: return pick<k0 + 1>(mc, sl, ht, counted);
-    case 2: return pick<k0 + 2>(mc, sl, ht, counted);
-    case 3: return pick<k0 + 3>(mc, sl, ht, counted);
-    case 4: return pick<k0 + 4>(mc, sl, ht, counted);
+    case 1: return pick<k0 + 1>(mc, sl, ht, counted, lt);
+    case 2: return pick<k0 + 2>(mc, sl, ht, counted, lt);
+    case 3: return pick<k0 + 3>(mc, sl, ht, counted, lt);
+    case 4: return pick<k0 + 4>(mc, sl, ht, counted, lt);
     default: return nullptr;
     }
 }

@@ -33,6 +33,7 @@ def test_stream_rejects_bad_params():
     (21, 48, 40, 70, 240, 30, 10, 4),     # the reference's 30/10 blocks, 3 blocks
     (22, 40, 32, 33, 150, 10, 5, 3),      # the served 10-frame windows, stride 5
     (23, 56, 40, 25, 200, 30, 10, 3),     # a single block
+    (24, 128, 96, 40, 30, 30, 10, 3),     # sparse blocks (s 25-29, steps 1 then 12-14): the fused first passes
 ])
 def test_stream_video_bit_exact(have_gpu, seed, W, H, D, K, B, O, iters):
     rgb, fwd, bwd, _ = asv.synth_video(seed, W, H, D, 6, 2.0, 4.0)
