@@ -547,6 +547,7 @@ int asv_upload_frames(asv_ctx* c, const uint8_t* rgb, void* stream)
     HIP_TRY(hipMemcpyAsync(c->rgb, rgb, (size_t)c->V() * 3, hipMemcpyDefault, S(stream)));
     c->have_frames = true;
     c->prepared = false;
+    c->lab_frames = c->lab_kept = 0;
     return ASV_OK;
 }
 
@@ -587,6 +588,14 @@ int asv_shift_frames(asv_ctx* c, int32_t src_frame, int32_t n_frames, void* stre
     HIP_TRY(hipMemcpyAsync(c->rgb, c->rgb + f0 * 3, n * 3, hipMemcpyDeviceToDevice, S(stream)));
     HIP_TRY(hipMemcpyAsync(c->fwd, c->fwd + f0, n * 8, hipMemcpyDeviceToDevice, S(stream)));
     HIP_TRY(hipMemcpyAsync(c->bwd, c->bwd + f0, n * 8, hipMemcpyDeviceToDevice, S(stream)));
+    // the moved frames' LAB moves with them when it is current (4 bytes a
+    // voxel copied instead of the blur and conversion of those frames)
+    if (c->lab_frames >= src_frame + n_frames) {
+        HIP_TRY(hipMemcpyAsync(c->lab, c->lab + f0, n * 4, hipMemcpyDeviceToDevice, S(stream)));
+        c->lab_frames = c->lab_kept = n_frames;
+    } else {
+        c->lab_frames = c->lab_kept = 0;
+    }
     c->prepared = false;
     return ASV_OK;
 }
@@ -601,6 +610,8 @@ int asv_upload_frames_range(asv_ctx* c, int32_t first, int32_t count, const uint
     HIP_TRY(hipMemcpyAsync(c->rgb + f0 * 3, rgb, n * 3, hipMemcpyDefault, S(stream)));
     HIP_TRY(hipMemcpyAsync(c->fwd + f0, fwd, n * 8, hipMemcpyDefault, S(stream)));
     HIP_TRY(hipMemcpyAsync(c->bwd + f0, bwd, n * 8, hipMemcpyDefault, S(stream)));
+    if (count > 0 && c->lab_frames > first) c->lab_frames = first;
+    if (c->lab_kept > c->lab_frames) c->lab_kept = c->lab_frames;
     c->have_frames = true;
     c->have_flow = true;
     c->prepared = false;
@@ -622,7 +633,18 @@ int asv_prepare(asv_ctx* c, void* stream)
     int r = check_ready(c);
     if (r) return r;
     hipStream_t st = S(stream);
-    HIP_TRY(launch_prepare(st, c->d, c->rgb, c->p.frames_prepared, c->lut, c->lab));
+    // frames [z0, D): all but the frames moved with their LAB (per frame,
+    // so the launch covers a suffix of the volume)
+    const int z0 = std::min(c->lab_kept, c->d.D);
+    if (z0 < c->d.D) {
+        const size_t plane = (size_t)c->d.W * c->d.H;
+        Dims rest = c->d;
+        rest.D = c->d.D - z0;
+        HIP_TRY(launch_prepare(st, rest, c->rgb + (size_t)z0 * plane * 3, c->p.frames_prepared, c->lut,
+                               c->lab + (size_t)z0 * plane));
+    }
+    c->lab_frames = c->d.D;
+    c->lab_kept = 0;
     // the seed stages of the run raise it when a metric needs the exact clamp
     HIP_TRY(launch_fill(st, c->unsafe, 1, c->force_exact ? 1 : 0));
     if (c->p.seed_init == ASV_INIT_DENSITY && c->gdims[0] > 0 && (r = density_reseed(c, st))) return r;

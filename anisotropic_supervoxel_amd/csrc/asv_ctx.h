@@ -20,6 +20,13 @@ struct asv_ctx {
     std::vector<int4> grid_staging;
     int64_t bytes = 0;
     bool have_frames = false, have_flow = false, prepared = false;
+    // Leading frames whose LAB matches the frames now on the device (a
+    // frame's LAB depends on that frame alone), and of them the frames that
+    // asv_shift_frames moved with their LAB since the last asv_prepare,
+    // which converts only the frames after those (every other prepare
+    // converts the whole volume)
+    int lab_frames = 0;
+    int lab_kept = 0;
     bool trace_on = false;
     int cur = 0;            // index of the label buffer holding the latest labels
     bool fresh = false;     // labels[cur] holds only the seed stage's scattered seeds
