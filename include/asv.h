@@ -102,7 +102,9 @@ int asv_create_seeded(const asv_params* p, const int32_t* xyz, int32_t n, int32_
 int asv_reseed(asv_ctx* ctx, const int32_t* xyz, int32_t n, int32_t grid_step, int32_t depth, void* stream);
 /* Device-side move of frames [src_frame, src_frame + n_frames) to [0, n_frames)
  * of the frame and flow buffers (the overlap kept between blocks,
- * streamGPU-AS/Render.cpp:545-556); ranges must not overlap. */
+ * streamGPU-AS/Render.cpp:545-556); ranges must not overlap.  When the
+ * moved frames' LAB is current it moves too, and the next asv_prepare
+ * converts only the frames after them. */
 int asv_shift_frames(asv_ctx* ctx, int32_t src_frame, int32_t n_frames, void* stream);
 /* Upload frames [first, first + count) of frames and both flows. */
 int asv_upload_frames_range(asv_ctx* ctx, int32_t first, int32_t count, const uint8_t* rgb, const float* fwd,
