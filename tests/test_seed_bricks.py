@@ -122,3 +122,39 @@ def test_fused_passes_benchmark_clip_first_iteration(have_gpu):
     rgb, fwd, bwd, _ = asv.synth_video(2, 640, 480, 200, 32, 4.0, 6.0)
     xyz, s = orc.grid_seeds(640, 480, 200, 10000)
     _check_fused_passes(rgb, fwd, bwd, xyz, s)
+
+
+def test_partial_floods_end_in_32bit_labels(have_gpu):
+    """A flood stopped after n passes (asv_flood_passes) for every n the
+    fused form allows: marching passes that feed another marching pass of
+    the same flood exchange 16-bit labels and the flood's last pass writes
+    32-bit labels, so each stopping point sees the first, middle and last
+    label widths in another combination.  Each partial flood equals the
+    oracle's first n passes over the seed volume (-1 voxels included)."""
+    W, H, D = 200, 160, 48
+    rgb, fwd, bwd, _ = asv.synth_video(33, W, H, D, 8, 2.0, 4.0)
+    xyz, s = orc.grid_seeds(W, H, D, 120)
+    A = len(xyz)
+    p = asv.default_params(W, H, D, A, lloyd_iters=2)
+    vol = _seed_volume(xyz, D, H, W)
+    ratio = np.float32(np.float32(p.compactness) / np.float32(s))
+    ratio = np.float32(ratio * ratio)
+    got = {}
+    with asv.Clip(p, seeds_xyz=xyz, grid_step=s) as c:
+        steps, mult = c.schedule()
+        assert steps[0] <= 2 and A * 729 * 3 <= W * H * D, "case must take the fused passes"
+        c.enable_trace()
+        c.upload(rgb, fwd, bwd)
+        c.prepare()
+        for n in range(2, len(steps) + 1):
+            c.seed_stage(1)
+            c.flood_passes(1, n)
+            got[n] = c.labels()        # (iteration 1's seed stage may run again)
+        lab = c.lab()
+        recs = _records(c.trace()[0])
+    want = vol
+    for i, k in enumerate(steps):
+        want = orc.jfa_pass(want, lab, recs, k, float(np.float32(mult[i]) * ratio))
+        if i + 1 >= 2:
+            bad = np.argwhere(got[i + 1] != want)
+            assert bad.size == 0, f"{len(bad)} mismatches after {i + 1} passes, first at {bad[:5].tolist()}"
