@@ -91,7 +91,10 @@ struct asv_ctx {
     // 16-bit labels between marching passes (launch_jfa's lt): pass i of a
     // flood of n passes writes them where the next pass of the same flood
     // marches in a form that reads them, so only the flood's last pass
-    // writes 32-bit labels.  `fresh_iter`: the passes of an iteration that
+    // writes 32-bit labels.  Passes of step 4 and more read 32-bit labels
+    // (16-bit input took the step-5 pass from 405 to 501 us at 640x480x200,
+    // while steps 1-3 gain; those forms are not instantiated above step 3).
+    // `fresh_iter`: the passes of an iteration that
     // began with the seed stage (pass_kind), else every pass is a full one.
     bool pass_label16(size_t i, bool fresh_iter) const
     {
@@ -100,8 +103,11 @@ struct asv_ctx {
     }
     int label_widths(size_t i, size_t n, bool fresh_iter) const
     {
-        const bool out16 = i + 1 < n && pass_label16(i, fresh_iter) && pass_label16(i + 1, fresh_iter);
-        const bool in16 = i > 0 && i < n && pass_label16(i - 1, fresh_iter) && pass_label16(i, fresh_iter);
+        auto writes16 = [&](size_t j) {
+            return j + 1 < n && steps[j + 1] <= 3 && pass_label16(j, fresh_iter) && pass_label16(j + 1, fresh_iter);
+        };
+        const bool out16 = writes16(i);
+        const bool in16 = i > 0 && i < n && writes16(i - 1);
         return in16 ? (out16 ? 2 : 3) : (out16 ? 1 : 0);
     }
 

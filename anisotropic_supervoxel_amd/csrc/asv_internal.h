@@ -81,7 +81,8 @@ hipError_t launch_zero_i32(hipStream_t st, int* p, int64_t n);
 hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t* labels);
 // lt: the label widths of a marching pass (0: 32-bit in and out; 1 32-bit
 // in, 16-bit out; 2 16-bit in and out; 3 16-bit in, 32-bit out; 16-bit
-// labels are 0xFFFF for -1), only where march_label16 allows them.
+// labels are 0xFFFF for -1), only where march_label16 allows them, and
+// 16-bit input only at steps 1-3.
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
                       unsigned long long* stats = nullptr, int lt = 0);

@@ -708,16 +708,19 @@ static MarchFn march_fn(int mc, int sl)
 
 // The 16-bit label forms exist for two slices per step, the direct table
 // and ordinary launches (lt: 1 32-bit in, 16-bit out; 2 both 16-bit; 3
-// 16-bit in, 32-bit out).
+// 16-bit in, 32-bit out), with 16-bit input only up to step 3.
 template <int K>
 static MarchFn march_fn16(int mc, int lt)
 {
-    switch (lt) {
-    case 1: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, false, true> : k_jfa_march<K, 63, 2, false, false, false, true>;
-    case 2: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, true> : k_jfa_march<K, 63, 2, false, false, true, true>;
-    case 3: return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, false> : k_jfa_march<K, 63, 2, false, false, true, false>;
-    default: return nullptr;
+    if (lt == 1)
+        return mc == 31 ? k_jfa_march<K, 31, 2, false, false, false, true> : k_jfa_march<K, 63, 2, false, false, false, true>;
+    if constexpr (K <= 3) {
+        if (lt == 2)
+            return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, true> : k_jfa_march<K, 63, 2, false, false, true, true>;
+        if (lt == 3)
+            return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, false> : k_jfa_march<K, 63, 2, false, false, true, false>;
     }
+    return nullptr;
 }
 
 // Instances of steps 4P+1 .. 4P+4 (defined by asv_march.hip built with
