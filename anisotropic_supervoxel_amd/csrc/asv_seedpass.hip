@@ -37,7 +37,6 @@ constexpr int kBrickX = 32, kBrickY = 16;
 constexpr int kBrickThreads = 512;
 constexpr int kBrickLds = 96;      // list entries staged in LDS at a time
 constexpr int kNear = 4;           // seeds within 2 k1 of a seed kept per entry
-constexpr int kScanPer = 8;        // counts per thread and round of the scan
 
 // One seed in a brick's list: id, voxel (x | y << 16, z) and its record.
 struct alignas(16) BrickEntry {
@@ -54,13 +53,17 @@ __device__ __forceinline__ void brick_span(int p, int R, int n, int B, int* lo, 
 }
 
 // Thread (seed i, u) takes the u-th brick of the at most mx x my x mz that
-// seed i's reach meets: with entries == nullptr it adds one to count[b],
-// else it writes the seed's entry at cursor[b] (and advances it).
+// seed i's reach meets: with entries == nullptr it adds one to count[b]
+// (and the first thread zeroes the running total k_brick_alloc takes the
+// lists' segments from), else it writes the seed's entry at cursor[b] (and
+// advances it).
 __global__ __launch_bounds__(256) void k_brick_bin(int W, int H, int D, int A, int R, int bz_dim, int nbx, int nby,
                                                    int mx, int my, int mz, const int4* __restrict__ pos,
                                                    const SeedRec* __restrict__ rec, int* __restrict__ count,
-                                                   int* __restrict__ cursor, BrickEntry* __restrict__ entries)
+                                                   int* __restrict__ cursor, BrickEntry* __restrict__ entries,
+                                                   int* __restrict__ total)
 {
+    if (!entries && blockIdx.x == 0 && threadIdx.x == 0) *total = 0;
     const int per = mx * my * mz;
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int i = (int)(g / per), u = (int)(g - (int64_t)i * per);
@@ -86,52 +89,32 @@ __global__ __launch_bounds__(256) void k_brick_bin(int W, int H, int D, int A, i
     entries[atomicAdd(&cursor[b], 1)] = e;
 }
 
-// One block of 1024 threads: exclusive prefix sums of count[0 .. nb) into
-// off (off[nb] the total) and cursor, and count back to zero for the next
-// binning.  Rounds of 1024 x kScanPer counts, each thread's kScanPer loaded
-// together.
-__global__ __launch_bounds__(1024) void k_brick_scan(int nb, int* __restrict__ count, int* __restrict__ off,
-                                                     int* __restrict__ cursor)
+// Thread b: brick b's list segment [off[b], off[b] + count[b]) of the
+// entries buffer, taken from the running total with one atomic per wave
+// (the segments' order in the buffer is arbitrary, as the entries' order
+// within a list is: the brick kernel's minima depend on neither); cursor[b]
+// starts at off[b], so after the filling pass it ends the list, and count[b]
+// returns to zero for the next binning.
+__global__ __launch_bounds__(256) void k_brick_alloc(int nb, int* __restrict__ count, int* __restrict__ off,
+                                                      int* __restrict__ cursor, int* __restrict__ total)
 {
-    __shared__ int s_w[16];
-    __shared__ int s_carry;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (t == 0) s_carry = 0;
-    __syncthreads();
-    for (int r0 = 0; r0 < nb; r0 += 1024 * kScanPer) {
-        const int b0 = r0 + t * kScanPer;
-        int c[kScanPer];
-        int sum = 0;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int c = b < nb ? count[b] : 0;
+    int incl = c;
 #pragma unroll
-        for (int u = 0; u < kScanPer; ++u) {
-            c[u] = b0 + u < nb ? count[b0 + u] : 0;
-            sum += c[u];
-        }
-        int incl = sum;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d);
-            if (lane >= d) incl += v;
-        }
-        if (lane == 63) s_w[wave] = incl;
-        __syncthreads();
-        int base = s_carry;
-        for (int w = 0; w < wave; ++w) base += s_w[w];
-        int run = base + incl - sum;
-#pragma unroll
-        for (int u = 0; u < kScanPer; ++u) {
-            if (b0 + u < nb) {
-                off[b0 + u] = run;
-                cursor[b0 + u] = run;
-                count[b0 + u] = 0;
-            }
-            run += c[u];
-        }
-        __syncthreads();                      // every thread has read s_carry and s_w
-        if (t == 1023) s_carry = run;
-        __syncthreads();
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
     }
-    if (t == 0) off[nb] = s_carry;
+    int base = 0;
+    if (lane == 63) base = atomicAdd(total, incl);
+    base = __shfl(base, 63);
+    if (b < nb) {
+        off[b] = base + incl - c;
+        cursor[b] = base + incl - c;
+        count[b] = 0;
+    }
 }
 
 // The coordinates c of [lo, hi] (inside [0, n - 1]) that have t among their
@@ -169,6 +152,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_seed_bricks(int W, int H, int
                                                                float ratio2, int nbx, int nby,
                                                                const uint32_t* __restrict__ lab,
                                                                const int* __restrict__ off,
+                                                               const int* __restrict__ end,
                                                                const BrickEntry* __restrict__ entries,
                                                                int32_t* __restrict__ out)
 {
@@ -190,7 +174,7 @@ __global__ __launch_bounds__(kBrickThreads) void k_seed_bricks(int W, int H, int
     const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
     const int x0 = bx * kBrickX, y0 = by * kBrickY, z0 = bz * BZ;
     const int x1 = min(x0 + kBrickX, W) - 1, y1 = min(y0 + kBrickY, H) - 1, z1 = min(z0 + BZ, D) - 1;
-    const int o0 = off[b], n = off[b + 1] - o0;
+    const int o0 = off[b], n = end[b] - o0;
     const BrickEntry* __restrict__ g_e = entries + o0;
 #pragma unroll
     for (int i = 0; i < kVox / kBrickThreads; ++i) s_key[i * kBrickThreads + tid] = ASV_KEY_INIT;
@@ -429,15 +413,19 @@ hipError_t launch_seed_bricks(hipStream_t st, const Dims& d, const int4* pos, co
     const int mx = (int)reach_bricks(R, kBrickX), my = (int)reach_bricks(R, kBrickY),
               mz = (int)reach_bricks(R, kSeedBrickZ);
     const unsigned g = cdiv((int64_t)d.A * mx * my * mz, 256);
+    // the running total of the lists' segments (zeroed by the counting
+    // pass; without seeds every list is empty and no entry is read)
+    int* total = bb.off + bb.nb_cap;
     if (d.A > 0)
         hipLaunchKernelGGL(k_brick_bin, dim3(g), dim3(256), 0, st, d.W, d.H, d.D, d.A, R, kSeedBrickZ, nbx, nby, mx,
-                           my, mz, pos, rec, bb.count, bb.cursor, (BrickEntry*)nullptr);
-    hipLaunchKernelGGL(k_brick_scan, dim3(1), dim3(1024), 0, st, (int)nb, bb.count, bb.off, bb.cursor);
+                           my, mz, pos, rec, bb.count, bb.cursor, (BrickEntry*)nullptr, total);
+    hipLaunchKernelGGL(k_brick_alloc, dim3(cdiv(nb, 256)), dim3(256), 0, st, (int)nb, bb.count, bb.off, bb.cursor,
+                       total);
     if (d.A > 0)
         hipLaunchKernelGGL(k_brick_bin, dim3(g), dim3(256), 0, st, d.W, d.H, d.D, d.A, R, kSeedBrickZ, nbx, nby, mx,
-                           my, mz, pos, rec, bb.count, bb.cursor, entries);
+                           my, mz, pos, rec, bb.count, bb.cursor, entries, total);
     hipLaunchKernelGGL(k_seed_bricks<kSeedBrickZ>, dim3((unsigned)nb), dim3(kBrickThreads), 0, st, d.W, d.H, d.D, k1,
-                       k2, ratio1, ratio2, nbx, nby, lab, bb.off, entries, out);
+                       k2, ratio1, ratio2, nbx, nby, lab, bb.off, bb.cursor, entries, out);
     return hipGetLastError();
 }
 
