@@ -1825,20 +1825,27 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
     return x ^ (x >> 31);
 }
 
+// One 64-bit atomic per block: every block's sum goes to the same word, so
+// the grid is kept to a few blocks per CU (with one atomic per wave over
+// 4096 blocks the adds serialised on that word: 155 us for a 1280x720x5
+// window's frames, against a few us of arithmetic).
 __global__ __launch_bounds__(256) void k_checksum(const int32_t* __restrict__ labels, int64_t n, int64_t index0,
                                                   unsigned long long* __restrict__ out)
 {
+    __shared__ unsigned long long s_w[4];
     unsigned long long acc = 0ull;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         acc += splitmix64(((unsigned long long)(index0 + i) << 32) | (unsigned)labels[i]);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, s_w[0] + s_w[1] + s_w[2] + s_w[3]);
 }
 
 hipError_t launch_checksum(hipStream_t st, const int32_t* labels, int64_t n, int64_t index0, unsigned long long* out)
 {
-    unsigned blocks = n > 0 ? cdiv(n, 256) : 1;
-    if (blocks > 4096u) blocks = 4096u;
+    unsigned blocks = n > 0 ? cdiv(n, 256 * 16) : 1;
+    if (blocks > 512u) blocks = 512u;
     hipLaunchKernelGGL(k_checksum, dim3(blocks), dim3(256), 0, st, labels, n, index0, out);
     return hipGetLastError();
 }
