@@ -6,6 +6,7 @@ import pytest
 
 import anisotropic_supervoxel_amd as asv
 from anisotropic_supervoxel_amd import api
+import oracle as orc
 import stream as ostream
 
 
@@ -44,3 +45,44 @@ def test_stream_video_bit_exact(have_gpu, seed, W, H, D, K, B, O, iters):
     bad = int((got != want).sum())
     assert bad == 0, f"{bad} stitched labels differ"
     assert (got >= 0).all()
+
+
+@pytest.mark.gpu
+def test_shifted_frames_keep_their_lab(have_gpu):
+    """asv_shift_frames moves the overlap frames' LAB with them and the next
+    prepare converts only the frames after them; frames re-uploaded inside
+    the moved range are converted again.  Each prepare's LAB volume equals
+    the oracle's LAB of the frames then on the device."""
+    from anisotropic_supervoxel_amd import _lib
+    W, H, D, O = 96, 64, 10, 5
+    a = asv.synth_video(40, W, H, D, 6, 2.0, 4.0)
+    b = asv.synth_video(41, W, H, D, 6, 2.0, 4.0)
+    rgb_a, fwd_a, bwd_a = (np.ascontiguousarray(x) for x in a[:3])
+    rgb_b, fwd_b, bwd_b = (np.ascontiguousarray(x) for x in b[:3])
+
+    def want(rgb):
+        return orc.lab_volume(orc.blur_flip(rgb))
+
+    p = asv.default_params(W, H, D, 40, lloyd_iters=1)
+    with asv.Clip(p) as c:
+        lib = c.lib
+        c.upload(rgb_a, fwd_a, bwd_a)
+        c.prepare()
+        assert np.array_equal(c.lab(), want(rgb_a))
+        # the last O frames become the first O; new frames after them
+        assert lib.asv_shift_frames(c.h, D - O, O, None) == 0
+        first = O
+        assert lib.asv_upload_frames_range(c.h, first, D - first, _lib.ptr(rgb_b[first:]), _lib.ptr(fwd_b[first:]),
+                                           _lib.ptr(bwd_b[first:]), None) == 0
+        c.prepare()
+        now = np.concatenate([rgb_a[D - O:], rgb_b[first:]])
+        assert np.array_equal(c.lab(), want(now))
+        # again, with frames 3 .. D re-uploaded: frames 3 and 4 of the moved
+        # range change and must be converted again
+        assert lib.asv_shift_frames(c.h, D - O, O, None) == 0
+        first = 3
+        assert lib.asv_upload_frames_range(c.h, first, D - first, _lib.ptr(rgb_a[first:]), _lib.ptr(fwd_a[first:]),
+                                           _lib.ptr(bwd_a[first:]), None) == 0
+        c.prepare()
+        now = np.concatenate([now[D - O:D - O + first], rgb_a[first:]])
+        assert np.array_equal(c.lab(), want(now))
