@@ -101,6 +101,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
 {
     constexpr int kInB = I16 ? 2 : 4;         // bytes per input label
     static_assert(S == 1 || S == 2, "one or two slices of a chain per step");
+    static_assert(kMarchTableIds <= 32768, "16-bit labels (direct table only) read sign-extended");
     using MaskT = typename MarchMask<MC>::T;
     constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
     constexpr int RA = R * R;
@@ -239,12 +240,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const bool exact = c.unsafe && *(gptr<const int>)c.unsafe;
 
         // a label at a frame base plus a byte offset, and a label stored at
-        // this thread's voxel of frame zk
+        // this thread's voxel of frame zk (a 16-bit label is read sign-
+        // extended: ids stay below 2^15, and 0xFFFF becomes -1 with no
+        // compare; 351 against 355 us per full pass at 640x480x200)
         auto ld_label = [&](gptr<const char> base, uint32_t off) -> int {
-            if (I16) {
-                const unsigned v = ld_off((gptr<const uint16_t>)base, off);
-                return v == 0xFFFFu ? -1 : (int)v;
-            }
+            if (I16) return (int)ld_off((gptr<const int16_t>)base, off);
             return ld_off((gptr<const int32_t>)base, off);
         };
         auto st_label = [&](int zk, int v) {
