@@ -158,7 +158,7 @@ int launch_group(const std::vector<asv_ctx*>& clips, const JfaClip* dev_desc, co
             break;
         case 3:                                     // both first passes, into the buffer pass 2 reads
             BHIP(launch_seed_bricks(st, c->d, c->pos, c->rec, c->lab, c->labels[0], c->steps[0], c->steps[1], ratio,
-                                    c->mult[1] * c->ratio, c->bricks));
+                                    c->mult[1] * c->ratio, c->bricks, c->writes_label16(1, c->steps.size(), true)));
             break;
         case 4:                                     // done by pass 0
             break;

@@ -706,7 +706,7 @@ hipError_t launch_pass(asv_ctx* c, size_t i, size_t n, hipStream_t st)
         // both passes: the second's output lands where it would after two
         // passes, in the buffer this pass reads
         return launch_seed_bricks(st, c->d, c->pos, c->rec, c->lab, c->labels[c->cur], c->steps[0], c->steps[1],
-                                  ratio, c->mult[1] * c->ratio, c->bricks);
+                                  ratio, c->mult[1] * c->ratio, c->bricks, c->writes_label16(1, n, true));
     case 4:
         return hipSuccess;
     default:

@@ -91,23 +91,28 @@ struct asv_ctx {
     // 16-bit labels between marching passes (launch_jfa's lt): pass i of a
     // flood of n passes writes them where the next pass of the same flood
     // marches in a form that reads them, so only the flood's last pass
-    // writes 32-bit labels.  Passes of step 4 and more read 32-bit labels
-    // (16-bit input took the step-5 pass from 405 to 501 us at 640x480x200,
-    // while steps 1-3 gain; those forms are not instantiated above step 3).
-    // `fresh_iter`: the passes of an iteration that
-    // began with the seed stage (pass_kind), else every pass is a full one.
+    // writes 32-bit labels.  Passes of step 1-5 read 16-bit labels (the
+    // schedules' steps after the second pass; the forms are not built for
+    // larger steps).  `fresh_iter`: the passes of an iteration that began
+    // with the seed stage (pass_kind), else every pass is a full one.
     bool pass_label16(size_t i, bool fresh_iter) const
     {
         return (fresh_iter ? pass_kind(i) : 0) == 0 &&
                asv::march_label16(d, steps[i], p.tie_rule, march_stats != nullptr);
     }
+    // Whether pass i's output is 16-bit: a marching pass followed by one that
+    // reads 16-bit labels, or the fused first passes' second (written by the
+    // first's launch, pass_kind 4) followed by one.
+    bool writes_label16(size_t i, size_t n, bool fresh_iter) const
+    {
+        if (i + 1 >= n || steps[i + 1] > 5 || !pass_label16(i + 1, fresh_iter)) return false;
+        const int k = fresh_iter ? pass_kind(i) : 0;
+        return k == 4 || (k == 0 && pass_label16(i, fresh_iter));
+    }
     int label_widths(size_t i, size_t n, bool fresh_iter) const
     {
-        auto writes16 = [&](size_t j) {
-            return j + 1 < n && steps[j + 1] <= 3 && pass_label16(j, fresh_iter) && pass_label16(j + 1, fresh_iter);
-        };
-        const bool out16 = writes16(i);
-        const bool in16 = i > 0 && i < n && writes16(i - 1);
+        const bool out16 = writes_label16(i, n, fresh_iter);
+        const bool in16 = i > 0 && i < n && writes_label16(i - 1, n, fresh_iter);
         return in16 ? (out16 ? 2 : 3) : (out16 ? 1 : 0);
     }
 

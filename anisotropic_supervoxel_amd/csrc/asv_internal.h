@@ -82,7 +82,7 @@ hipError_t launch_place(hipStream_t st, const Dims& d, const int4* pos, int32_t*
 // lt: the label widths of a marching pass (0: 32-bit in and out; 1 32-bit
 // in, 16-bit out; 2 16-bit in and out; 3 16-bit in, 32-bit out; 16-bit
 // labels are 0xFFFF for -1), only where march_label16 allows them, and
-// 16-bit input only at steps 1-3.
+// 16-bit input only at steps 1-5.
 hipError_t launch_jfa(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out,
                       const uint32_t* lab, const SeedRec* rec, int k, float ratio, int tie_rule, const int* unsafe,
                       unsigned long long* stats = nullptr, int lt = 0);
@@ -132,8 +132,11 @@ void seed_brick_grid(int W, int H, int D, int* nbx, int* nby, int* nbz);
 int64_t seed_brick_bytes(int W, int H, int D, int64_t seeds, int R);
 hipError_t alloc_seed_bricks(SeedBricks* bb, int W, int H, int D, int seeds, int R, int64_t* total);
 void free_seed_bricks(SeedBricks* bb);
+// (out16: 16-bit output labels, 0xFFFF for -1, for a marching pass that
+// reads them; at most kMarchTableIds seeds)
 hipError_t launch_seed_bricks(hipStream_t st, const Dims& d, const int4* pos, const SeedRec* rec, const uint32_t* lab,
-                              int32_t* out, int k1, int k2, float ratio1, float ratio2, const SeedBricks& bb);
+                              int32_t* out, int k1, int k2, float ratio1, float ratio2, const SeedBricks& bb,
+                              bool out16 = false);
 
 // How pass i of an iteration that began with the seed stage runs, for a
 // schedule `steps`, A seeds and V voxels: 1 the first pass over the

@@ -708,13 +708,13 @@ static MarchFn march_fn(int mc, int sl)
 
 // The 16-bit label forms exist for two slices per step, the direct table
 // and ordinary launches (lt: 1 32-bit in, 16-bit out; 2 both 16-bit; 3
-// 16-bit in, 32-bit out), with 16-bit input only up to step 3.
+// 16-bit in, 32-bit out), with 16-bit input only up to step 5.
 template <int K>
 static MarchFn march_fn16(int mc, int lt)
 {
     if (lt == 1)
         return mc == 31 ? k_jfa_march<K, 31, 2, false, false, false, true> : k_jfa_march<K, 63, 2, false, false, false, true>;
-    if constexpr (K <= 3) {
+    if constexpr (K <= 5) {
         if (lt == 2)
             return mc == 31 ? k_jfa_march<K, 31, 2, false, false, true, true> : k_jfa_march<K, 63, 2, false, false, true, true>;
         if (lt == 3)

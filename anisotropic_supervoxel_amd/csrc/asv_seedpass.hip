@@ -146,8 +146,9 @@ __device__ __forceinline__ int ax_first(int a) { return a & 0xffff; }
 __device__ __forceinline__ int ax_stride(int a) { return (a >> 16) & 31; }
 __device__ __forceinline__ int ax_count(int a) { return a >> 21; }
 
-// Brick kernel: BZ frames deep, kBrickX x kBrickY voxels per frame.
-template <int BZ>
+// Brick kernel: BZ frames deep, kBrickX x kBrickY voxels per frame; O16:
+// 16-bit output labels (0xFFFF for -1) for a marching pass that reads them.
+template <int BZ, bool O16>
 __global__ __launch_bounds__(kBrickThreads) void k_seed_bricks(int W, int H, int D, int k1, int k2, float ratio1,
                                                                float ratio2, int nbx, int nby,
                                                                const uint32_t* __restrict__ lab,
@@ -338,13 +339,17 @@ __global__ __launch_bounds__(kBrickThreads) void k_seed_bricks(int W, int H, int
     const int x = x0 + tid % kBrickX, yz = tid / kBrickX;
     const int y = y0 + yz % kBrickY, zs = yz / kBrickY;
     if (x <= x1 && y <= y1) {
-        int32_t* o = out + ((size_t)(z0 + zs) * H + y) * W + x;
+        const size_t v0 = ((size_t)(z0 + zs) * H + y) * W + x;
         const size_t step = (size_t)(kRows / kBrickY) * H * W;
 #pragma unroll
         for (int i = 0; i < kVox / kBrickThreads; ++i) {
             if (z0 + zs + i * (kRows / kBrickY) <= z1) {
                 const unsigned long long kk = s_key[i * kBrickThreads + tid];
-                o[i * step] = kk < ASV_KEY_INIT ? (int32_t)(unsigned)(kk & 0xffffffffull) : -1;
+                const int32_t l = kk < ASV_KEY_INIT ? (int32_t)(unsigned)(kk & 0xffffffffull) : -1;
+                if (O16)
+                    reinterpret_cast<uint16_t*>(out)[v0 + i * step] = (uint16_t)l;
+                else
+                    out[v0 + i * step] = l;
             }
         }
     }
@@ -399,7 +404,7 @@ void free_seed_bricks(SeedBricks* bb)
 }
 
 hipError_t launch_seed_bricks(hipStream_t st, const Dims& d, const int4* pos, const SeedRec* rec, const uint32_t* lab,
-                              int32_t* out, int k1, int k2, float ratio1, float ratio2, const SeedBricks& bb)
+                              int32_t* out, int k1, int k2, float ratio1, float ratio2, const SeedBricks& bb, bool out16)
 {
     if (k1 < 1 || k1 > kSeedFuseMaxK1 || k2 < 1 || k2 > kSeedPassMaxStep || d.W >= 32768 || d.H >= 32768 ||
         d.D >= 32768)
@@ -424,8 +429,13 @@ hipError_t launch_seed_bricks(hipStream_t st, const Dims& d, const int4* pos, co
     if (d.A > 0)
         hipLaunchKernelGGL(k_brick_bin, dim3(g), dim3(256), 0, st, d.W, d.H, d.D, d.A, R, kSeedBrickZ, nbx, nby, mx,
                            my, mz, pos, rec, bb.count, bb.cursor, entries, total);
-    hipLaunchKernelGGL(k_seed_bricks<kSeedBrickZ>, dim3((unsigned)nb), dim3(kBrickThreads), 0, st, d.W, d.H, d.D, k1,
-                       k2, ratio1, ratio2, nbx, nby, lab, bb.off, bb.cursor, entries, out);
+    if (out16 && d.A > kMarchTableIds) return hipErrorInvalidValue;   // ids must fit 16 bits
+    if (out16)
+        hipLaunchKernelGGL((k_seed_bricks<kSeedBrickZ, true>), dim3((unsigned)nb), dim3(kBrickThreads), 0, st, d.W, d.H,
+                           d.D, k1, k2, ratio1, ratio2, nbx, nby, lab, bb.off, bb.cursor, entries, out);
+    else
+        hipLaunchKernelGGL((k_seed_bricks<kSeedBrickZ, false>), dim3((unsigned)nb), dim3(kBrickThreads), 0, st, d.W,
+                           d.H, d.D, k1, k2, ratio1, ratio2, nbx, nby, lab, bb.off, bb.cursor, entries, out);
     return hipGetLastError();
 }
 
