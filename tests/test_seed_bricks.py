@@ -126,14 +126,16 @@ def test_fused_passes_benchmark_clip_first_iteration(have_gpu):
 
 @pytest.mark.parametrize("W,H,D,K,vseed", [
     (200, 160, 48, 120, 33),     # steps 1, 11, 6, 3, 2, 1, 2, 1
-    (128, 96, 30, 15, 34),       # steps 1, 14, 7, 4, 2, 1, 2, 1: a step-4 pass reads 32-bit labels
+    (128, 96, 30, 15, 34),       # steps 1, 14, 7, 4, 2, 1, 2, 1: the step-7 pass reads 32-bit labels
+    (200, 160, 48, 250, 35),     # steps 1, 9, 5, 3, 2, 1, 2, 1: the fused passes write 16-bit labels
+    (200, 160, 48, 300, 36),     # steps 1, 8, 4, 2, 1, 2, 1
 ])
 def test_partial_floods_end_in_32bit_labels(have_gpu, W, H, D, K, vseed):
     """A flood stopped after n passes (asv_flood_passes) for every n the
-    fused form allows: marching passes that feed another marching pass of
-    step 1-3 in the same flood exchange 16-bit labels and the flood's last
-    pass writes 32-bit labels, so each stopping point sees the first, middle
-    and last label widths in another combination.  Each partial flood equals
+    fused form allows: the fused passes and marching passes that feed a
+    marching pass of step 1-5 in the same flood hand it 16-bit labels and
+    the flood's last pass writes 32-bit labels, so each stopping point sees
+    the first, middle and last label widths in another combination.  Each partial flood equals
     the oracle's first n passes over the seed volume (-1 voxels included)."""
     rgb, fwd, bwd, _ = asv.synth_video(vseed, W, H, D, 8, 2.0, 4.0)
     xyz, s = orc.grid_seeds(W, H, D, K)
