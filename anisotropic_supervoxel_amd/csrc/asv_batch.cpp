@@ -76,8 +76,9 @@ JfaClip descriptor(const asv_ctx* c, int p)
     d.H = c->d.H;
     d.D = c->d.D;
     d.A = c->d.A;
-    d.tiles_x = (c->d.W + 15) / 16;
-    d.tiles_y = (c->d.H + 15) / 16;
+    const int tw = march_tile_w(c->steps[(size_t)p]);
+    d.tiles_x = (c->d.W + tw - 1) / tw;
+    d.tiles_y = (c->d.H + kMarchTileH - 1) / kMarchTileH;
     d.ratio = c->mult[(size_t)p] * c->ratio;
     d.unsafe = c->unsafe;
     d.stats = c->march_stats;

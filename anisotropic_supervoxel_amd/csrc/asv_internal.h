@@ -165,6 +165,14 @@ hipError_t launch_jfa_seeds(hipStream_t st, const Dims& d, const int32_t* in, in
 // device memory for launch_march_batch.  march_max_grid is the resident
 // block count for `dyn` dynamic LDS bytes.
 constexpr int kMarchTableIds = 16384;       // byte-table slots (dynamic LDS <= 16 KiB)
+// Columns of the marching pass at step k: 16 rows of march_tile_w(k) voxels,
+// one thread per voxel of a slice.  From step 4 the columns are 32 wide,
+// which stages fewer halo cells per voxel (the halo is k on each side):
+// measured at 640x480x200 and 320x240x85, step 4 145 -> 139 us, step 5
+// 393 -> 379, step 7 210 -> 181, while step 2 went 319 -> 350 and step 3
+// 472 -> 504 (longer candidate lists per column) and step 1 barely moved.
+constexpr int march_tile_w(int k) { return k >= 4 ? 32 : 16; }
+constexpr int kMarchTileH = 16;
 constexpr int kMarchMaxIds = (1 << 30) - 1;  // ids below the resolved-index tag
 // ht: some clip of the launch has more than kMarchTableIds seeds (the
 // kernel whose table slots are shared and checked); counted: some clip has

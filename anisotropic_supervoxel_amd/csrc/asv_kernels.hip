@@ -1519,13 +1519,15 @@ static MarchFn march_kernel(int k, int mc, int sl, bool ht = false, bool counted
     }
 }
 
-// Dynamic LDS of the marching pass: the byte-per-id table for ids < amax,
-// in whole KiB, and at least the 4 KiB a wide step keeps its keys in.
-static int march_dyn_bytes(int amax)
+// Dynamic LDS of the marching pass at step k: the byte-per-id table for
+// ids < amax, in whole KiB, and at least what a wide step keeps its keys in
+// (two words per thread and slice: 4 KiB for 256 threads, 8 KiB for 512).
+static int march_dyn_bytes(int k, int amax)
 {
     const int bytes = ((amax > 0 ? amax : 1) + 3) / 4 * 4;
     const int kib = (bytes + 1023) / 1024 * 1024;
-    return kib > 4096 ? kib : 4096;
+    const int keys = 2 * 2 * march_tile_w(k) * kMarchTileH * 4;
+    return kib > keys ? kib : keys;
 }
 
 // Resident blocks of the marching pass (CUs x blocks per CU, a multiple of
@@ -1542,7 +1544,7 @@ int march_max_grid(int k, int dyn, bool ht, bool dense, int lt)
         int dev = 0, cus = 0, per_cu = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl, ht, false, lt), 256,
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)march_kernel(k, mc, sl, ht, false, lt), march_tile_w(k) * kMarchTileH,
                                                            (size_t)dyn);
         g = cus * (per_cu > 0 ? per_cu : 1);
         g = (g / 8) * 8;
@@ -1586,7 +1588,7 @@ int march_plan(int k, JfaClip* c, int n, int per_block, int grid_limit, int seg_
     int amax = 1;
     for (int i = 0; i < n; ++i) amax = c[i].A > amax ? c[i].A : amax;
     if (amax > kMarchMaxIds) return 0;
-    const int lds = march_dyn_bytes(amax < kMarchTableIds ? amax : kMarchTableIds);
+    const int lds = march_dyn_bytes(k, amax < kMarchTableIds ? amax : kMarchTableIds);
     int gmax = march_max_grid(k, lds, amax > kMarchTableIds, march_dense(c, n), lt);
     if (gmax <= 0) return 0;
     if (grid_limit > 0 && grid_limit < gmax) gmax = grid_limit < 8 ? 8 : (grid_limit / 8) * 8;
@@ -1631,7 +1633,8 @@ hipError_t launch_march(hipStream_t st, int k, const JfaClip& one, const JfaClip
     const JfaClip* a1 = dev;
     int a2 = n, a3 = nitems, a4 = dyn / 4;
     void* args[] = {(void*)&a0, (void*)&a1, (void*)&a2, (void*)&a3, (void*)&a4};
-    return hipLaunchKernel((const void*)fn, dim3((unsigned)grid), dim3(256), args, (size_t)dyn, st);
+    return hipLaunchKernel((const void*)fn, dim3((unsigned)grid), dim3((unsigned)(march_tile_w(k) * kMarchTileH)), args,
+                           (size_t)dyn, st);
 }
 
 hipError_t launch_march_batch(hipStream_t st, int k, const JfaClip* dev, int n, int grid, int nitems, int dyn,
@@ -1645,9 +1648,9 @@ template <int K>
 static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, int32_t* out, const uint32_t* lab,
                               const SeedRec* rec, float ratio, const int* unsafe, unsigned long long* stats, int lt)
 {
-    int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     if (d.A <= kMarchMaxIds) {
-        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, tx, ty, 1, 1, 0, ratio, unsafe, stats};
+        const int mx = (d.W + march_tile_w(K) - 1) / march_tile_w(K), my = (d.H + kMarchTileH - 1) / kMarchTileH;
+        JfaClip c{in, out, lab, rec, d.W, d.H, d.D, d.A, mx, my, 1, 1, 0, ratio, unsafe, stats};
         int nitems = 0, dyn = 0;
         const int g = march_plan(K, &c, 1, 0, 0, 0, &nitems, &dyn, lt);
         if (g > 0)
@@ -1655,6 +1658,7 @@ static hipError_t launch_tile(hipStream_t st, const Dims& d, const int32_t* in, 
                                 march_dense(&c, 1), lt);
     }
     if (lt != 0) return hipErrorInvalidValue;   // 16-bit labels only between marching passes
+    const int tx = (d.W + kTile - 1) / kTile, ty = (d.H + kTile - 1) / kTile;
     int nt = tx * ty * d.D;
     hipLaunchKernelGGL(k_jfa_tile<K>, dim3((unsigned)nt), dim3(256), 0, st, d.W, d.H, d.D, tx, ty, nt, in, out, lab,
                        rec, ratio);

@@ -96,16 +96,18 @@ __device__ __forceinline__ int march_len(const JfaClip& c, int g, int r)
 // passes move 2 bytes per label instead of 4 (the slabs with their halos,
 // and the output), and only the last pass of a flood writes 32-bit labels.
 template <int K, int MC, int S, bool HT, bool ST, bool I16, bool O16>
-__global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
+__global__ __launch_bounds__(512) void k_jfa_march(JfaClip one, const JfaClip* __restrict__ clips, int nclips,
                                                    int nitems, int slot_words)
 {
     constexpr int kInB = I16 ? 2 : 4;         // bytes per input label
     static_assert(S == 1 || S == 2, "one or two slices of a chain per step");
     static_assert(kMarchTableIds <= 32768, "16-bit labels (direct table only) read sign-extended");
     using MaskT = typename MarchMask<MC>::T;
-    constexpr int R = kTile + 2 * K;          // staged rows and columns per slab
-    constexpr int RA = R * R;
-    constexpr int NRS = (RA + 255) / 256;     // staged elements per thread and slab
+    constexpr int TW = march_tile_w(K);       // column width (and kMarchTileH rows)
+    constexpr int NT = TW * kMarchTileH;      // threads: one voxel of the slice each
+    constexpr int R = TW + 2 * K;             // staged columns per slab row
+    constexpr int RA = R * (kMarchTileH + 2 * K);
+    constexpr int NRS = (RA + NT - 1) / NT;   // staged elements per thread and slab
     constexpr MaskT kBadBit = (MaskT)1 << MC;
     // LAB of the next slice loaded during this one; at K = 2 the extra
     // register costs a block per CU, more than the latency it hides
@@ -136,14 +138,14 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // ST: the item's wide steps, their chunks, per-tap steps, candidate
     // evaluations of the one-mask path, its wave rounds and wave-slices
     __shared__ unsigned s_cnt[6];
-    __shared__ unsigned s_wmax[4];
+    __shared__ unsigned s_wmax[NT / 64];
     // ST: cycles the waves spent in the phases of steps that extend a table
     // generation (claims, conversion and staging, masks, prefetch issue,
     // evaluation), and those wave-steps
     __shared__ unsigned long long s_tcyc[6];
 
     const int tid = threadIdx.x;
-    for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
+    for (int i = tid; i < slot_words; i += NT) s_slot[i] = 0xFFFFFFFFu;
     if (tid == 0) {
         s_count = 0;
         s_coll = -1;
@@ -162,11 +164,11 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     const int i_end = i_beg + qx + (xcd < rx ? 1 : 0);
 
     const int wave = tid >> 6, lane = tid & 63;
-    // a wave covers four full rows of the column (64-byte label rows,
-    // against 32-byte rows of an 8 x 8 wave: 318 -> 308 us per pass at
-    // step 1, 419 -> 411 at step 5, 640x480x200)
-    const int lx = lane & 15;
-    const int ly = wave * 4 + (lane >> 4);
+    // a wave covers four rows of 16 voxels (64-byte label rows, against
+    // 32-byte rows of an 8 x 8 wave: 318 -> 308 us per pass at step 1,
+    // 419 -> 411 at step 5, 640x480x200), TW / 16 of them across a row
+    const int lx = (wave % (TW / 16)) * 16 + (lane & 15);
+    const int ly = (wave / (TW / 16)) * 4 + (lane >> 4);
     const int mbase = (ly + K) * R + (lx + K);
 
     int known = 0;                            // s_count as of the last barrier
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
     // next release clears the whole table (uniform)
     bool scrubbed = false;
     // key words of a wide step: two per thread and slice
-    constexpr int kKeyWords = 2 * 256 * S;
+    constexpr int kKeyWords = 2 * NT * S;
     // Wide steps (chunked lists) where their registers keep the occupancy:
     // at K = 2, and in the hashed form above K = 3, they cost a wave per
     // SIMD, and those steps evaluate per tap instead
@@ -214,17 +216,17 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         const gptr<const float> g_rec = (gptr<const float>)c.rec;
         const float ratio = c.ratio;
         const int tx = t % c.tiles_x, ty = t / c.tiles_x;
-        const int x0 = tx * kTile, y0 = ty * kTile;
+        const int x0 = tx * TW, y0 = ty * kMarchTileH;
         const bool hashed = HT && c.A > kMarchTableIds;   // uniform per item
         // a slab is staged in raster order over the grown tile: thread tid
-        // holds elements tid, tid + 256, ... (offsets within a frame, -1 for
+        // holds elements tid, tid + NT, ... (offsets within a frame, -1 for
         // none), so every lane of every wave stages
         // (byte offsets within a frame: a load takes the frame's base in
         // scalar registers and this offset)
         int soff[NRS];
 #pragma unroll
         for (int i = 0; i < NRS; ++i) {
-            const int f = tid + 256 * i;
+            const int f = tid + NT * i;
             const int rr = f / R, cc = f - (f / R) * R;
             soff[i] = f < RA ? kInB * (clampi(y0 - K + rr, 0, H - 1) * W + clampi(x0 - K + cc, 0, W - 1)) : -1;
         }
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         auto convert = [&](const int* ids, int slot, unsigned none) {
 #pragma unroll
             for (int i = 0; i < NRS; ++i) {
-                const int f = tid + 256 * i;
+                const int f = tid + NT * i;
                 const int v = ids[i];
                 if (f < RA) {
                     unsigned cix = none;
@@ -411,7 +413,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
         // release every claimed byte and empty the list (callers barrier after)
         auto release = [&]() {
             if (!scrubbed && known <= kListCap) {
-                for (int e = tid; e < known; e += 256) {
+                for (int e = tid; e < known; e += NT) {
                     const int sid = HT ? s_list[e] & (kMarchTableIds - 1) : s_list[e];
                     atomicOr(&s_slot[sid >> 2], 0xFFu << ((unsigned)(sid & 3) * 8u));
                 }
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                 // after a wide step every lane has read its last key words
                 // before any are cleared (scrubbed is uniform)
                 if (scrubbed) __syncthreads();
-                for (int i = tid; i < slot_words; i += 256) s_slot[i] = 0xFFFFFFFFu;
+                for (int i = tid; i < slot_words; i += NT) s_slot[i] = 0xFFFFFFFFu;
             }
         };
 
@@ -642,7 +644,7 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
             if (kWide && nchunk > 1) {
                 // a wide step: each slice's best key lowered chunk by chunk,
                 // kept between chunks in the table's words (this thread's
-                // own: word k 512 + tid low, k 512 + 256 + tid high)
+                // own: word 2 k NT + tid low, 2 k NT + NT + tid high)
                 for (int ch = 0;;) {
                     const bool last = ch + 1 == nchunk;
                     if (mine) {
@@ -651,13 +653,13 @@ __global__ __launch_bounds__(256) void k_jfa_march(JfaClip one, const JfaClip* _
                             if (k >= ns) break;
                             float l0, l1, l2;
                             lab_of(pk_here[k], l0, l1, l2);
-                            uint32_t* kw = s_slot + k * 512 + tid;
+                            uint32_t* kw = s_slot + k * 2 * NT + tid;
                             unsigned long long best = ASV_KEY_INIT;
-                            if (ch > 0) best = ((unsigned long long)kw[256] << 32) | kw[0];
+                            if (ch > 0) best = ((unsigned long long)kw[NT] << 32) | kw[0];
                             best = eval_masks(k, l0, l1, l2, best);
                             if (!last) {
                                 kw[0] = (uint32_t)best;
-                                kw[256] = (uint32_t)(best >> 32);
+                                kw[NT] = (uint32_t)(best >> 32);
                             } else {
                                 st_label(z + k * K,
                                          best == ASV_KEY_INIT ? -1 : (int32_t)(unsigned)(best & 0xffffffffull));

@@ -97,7 +97,8 @@ def test_counting_kernel_matches_plain(have_gpu):
 def test_benchmark_clip_step_paths(have_gpu):
     """The 640x480x200 benchmark clip (K=10000, marching steps 5, 3, 2, 1,
     2, 1): every step of each step size is counted (two slices per step,
-    1200 columns of 16 x 16), none needs more than one candidate mask, and
+    1200 columns of 16 x 16 below step 4, 600 of 32 x 16 from step 4), none
+    needs more than one candidate mask, and
     the counting kernel's labels are the ordinary kernel's (whose labels the
     full-size parity test checks against the oracle)."""
     rgb, fwd, bwd, _ = asv.synth_video(2, 640, 480, 200, 32, 4.0, 6.0)
@@ -118,6 +119,7 @@ def test_benchmark_clip_step_paths(have_gpu):
     assert sorted(st["by_step"]) == [1, 2, 3, 5]
     for k, r in st["by_step"].items():
         passes = 2 if k in (1, 2) else 1
-        assert r["steps"] >= 2 * passes * 1200 * 100, k
+        columns = (640 // (32 if k >= 4 else 16)) * (480 // 16)
+        assert r["steps"] >= 2 * passes * columns * 100, k
         assert r["wide"] == 0 and r["per_tap"] == 0
         assert r["candidates"] <= 64 * r["wave_rounds"]
